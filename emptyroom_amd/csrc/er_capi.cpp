@@ -1,0 +1,403 @@
+// er_capi.cpp -- extern "C" entry points declared in include/emptyroom.h.
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+
+#include "../../include/emptyroom.h"
+#include "er_host.h"
+#include "er_runtime.h"
+
+using er::ST_ARG;
+using er::ST_IO;
+using er::ST_OK;
+
+struct er_host_run {
+    int mode = 0;
+    er::host::GridParams gp;
+    er::host::HenryParams hp;
+    er::host::RunSetup setup;
+};
+
+namespace {
+
+er::Context *C(er_context *c) { return reinterpret_cast<er::Context *>(c); }
+const er::Context *C(const er_context *c) { return reinterpret_cast<const er::Context *>(c); }
+er::FrameworkDev *F(er_framework *f) { return reinterpret_cast<er::FrameworkDev *>(f); }
+er::GridDev *G(er_grid *g) { return reinterpret_cast<er::GridDev *>(g); }
+const er::GridDev *G(const er_grid *g) { return reinterpret_cast<const er::GridDev *>(g); }
+er::Batch *B(er_batch *b) { return reinterpret_cast<er::Batch *>(b); }
+
+int arg_error(const char *msg) {
+    er::set_error(msg);
+    return ST_ARG;
+}
+
+template <class Fn>
+int guarded(Fn fn) {
+    try {
+        return fn();
+    } catch (const std::bad_alloc &) {
+        er::set_error("host allocation failed");
+        return er::ST_NOMEM;
+    } catch (...) {
+        er::set_error("unexpected exception inside libemptyroom");
+        return er::ST_STATE;
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *er_last_error(void) { return er::last_error(); }
+const char *er_version(void) { return "emptyroom-mi355x 0.1 (gfx950)"; }
+
+int er_context_create(int device, size_t hbm_cap_bytes, er_context **out) {
+    if (!out) return arg_error("er_context_create: out is null");
+    return guarded([&]() -> int {
+        er::Context *c = nullptr;
+        int st = er::context_create(device, hbm_cap_bytes, &c);
+        if (st == ST_OK) *out = reinterpret_cast<er_context *>(c);
+        return st;
+    });
+}
+
+int er_context_destroy(er_context *ctx) {
+    delete C(ctx);
+    return ST_OK;
+}
+
+int er_context_get_tuning(const er_context *ctx, er_tuning *out) {
+    if (!ctx || !out) return arg_error("er_context_get_tuning: null argument");
+    const er::Tuning &t = C(ctx)->tuning;
+    out->points_per_lane = t.ppl;
+    out->lds_records = t.cap;
+    for (int d = 0; d < 3; d++) out->tile[d] = t.tile[d];
+    out->widom_per_bin = t.widom_per_bin;
+    out->s2_switch = t.s2_switch;
+    return ST_OK;
+}
+
+int er_context_set_tuning(er_context *ctx, const er_tuning *in) {
+    if (!ctx || !in) return arg_error("er_context_set_tuning: null argument");
+    if (in->points_per_lane != 1 && in->points_per_lane != 2)
+        return arg_error("er_context_set_tuning: points_per_lane must be 1 or 2");
+    if (in->tile[0] < 1 || in->tile[1] < 1 || in->tile[2] < 1 ||
+        in->tile[0] * in->tile[1] * in->tile[2] != 256 * in->points_per_lane)
+        return arg_error("er_context_set_tuning: tile must multiply to 256 * points_per_lane");
+    if (in->lds_records < 512 || in->lds_records > 4800)
+        return arg_error("er_context_set_tuning: lds_records must be in [512, 4800]");
+    if (in->widom_per_bin < 1) return arg_error("er_context_set_tuning: widom_per_bin < 1");
+    if (!(in->s2_switch > 0.0) || !(in->s2_switch <= 1.0))
+        return arg_error("er_context_set_tuning: s2_switch must be in (0, 1]");
+    er::Tuning &t = C(ctx)->tuning;
+    t.ppl = in->points_per_lane;
+    t.cap = in->lds_records;
+    for (int d = 0; d < 3; d++) t.tile[d] = in->tile[d];
+    t.widom_per_bin = in->widom_per_bin;
+    t.s2_switch = in->s2_switch;
+    return ST_OK;
+}
+
+int er_context_memory(const er_context *ctx, size_t *in_use, size_t *reserved) {
+    if (!ctx) return arg_error("er_context_memory: null context");
+    er::Context *c = const_cast<er::Context *>(C(ctx));
+    if (in_use) *in_use = c->pool.in_use();
+    if (reserved) *reserved = c->pool.reserved();
+    return ST_OK;
+}
+
+int er_framework_upload(er_context *ctx, const er_particle *atoms, int n_atoms,
+                        const double t_matrix[9], const int rep[3], double rc2,
+                        er_framework **out) {
+    if (!ctx || !out) return arg_error("er_framework_upload: null argument");
+    static_assert(sizeof(er_particle) == sizeof(er::Particle), "particle layout");
+    return guarded([&]() -> int {
+        er::FrameworkDev *f = nullptr;
+        int st = er::framework_upload(C(ctx), reinterpret_cast<const er::Particle *>(atoms),
+                                      n_atoms, t_matrix, rep, rc2, &f);
+        if (st == ST_OK) *out = reinterpret_cast<er_framework *>(f);
+        return st;
+    });
+}
+
+int er_framework_release(er_framework *fw) {
+    delete F(fw);
+    return ST_OK;
+}
+
+int er_grid_compute(er_context *ctx, er_framework *fw, const int n[3], int precision,
+                    void *host_out, er_grid **resident_out) {
+    if (!ctx || !fw || !n) return arg_error("er_grid_compute: null argument");
+    return guarded([&]() -> int {
+        er::GridDev *g = nullptr;
+        int st = er::grid_compute(C(ctx), F(fw), n, precision, host_out,
+                                  resident_out ? &g : nullptr);
+        if (st == ST_OK && resident_out) *resident_out = reinterpret_cast<er_grid *>(g);
+        return st;
+    });
+}
+
+int er_grid_download(er_grid *grid, void *host_out) {
+    if (!grid || !host_out) return arg_error("er_grid_download: null argument");
+    er::GridDev *g = G(grid);
+    if (hipSetDevice(g->ctx->device) != hipSuccess ||
+        hipMemcpyAsync(host_out, g->d_data, g->bytes(), hipMemcpyDeviceToHost, g->ctx->stream) !=
+            hipSuccess ||
+        hipStreamSynchronize(g->ctx->stream) != hipSuccess) {
+        er::set_error("er_grid_download: copy failed");
+        return er::ST_HIP;
+    }
+    return ST_OK;
+}
+
+int er_grid_info(const er_grid *grid, int n[3], int *precision) {
+    if (!grid) return arg_error("er_grid_info: null grid");
+    const er::GridDev *g = G(grid);
+    if (n)
+        for (int d = 0; d < 3; d++) n[d] = g->n[d];
+    if (precision) *precision = g->precision;
+    return ST_OK;
+}
+
+int er_grid_release(er_grid *grid) {
+    delete G(grid);
+    return ST_OK;
+}
+
+int er_widom(er_context *ctx, er_framework *fw, uint64_t n_insertions, double temperature,
+             uint64_t seed, er_widom_result *out, double *energies_out) {
+    if (!ctx || !fw || !out) return arg_error("er_widom: null argument");
+    return guarded([&]() -> int {
+        er::WidomResult r;
+        int st = er::widom_run(C(ctx), F(fw), n_insertions, temperature, seed, &r, energies_out);
+        if (st == ST_OK) {
+            out->sum_b = r.sum_b;
+            out->sum_eb = r.sum_eb;
+            out->n = r.n;
+        }
+        return st;
+    });
+}
+
+int er_count_incut_grid(er_context *ctx, er_framework *fw, const int n[3], uint64_t *out) {
+    if (!ctx || !fw || !n || !out) return arg_error("er_count_incut_grid: null argument");
+    return guarded([&]() -> int {
+        unsigned long long v = 0;
+        int st = er::count_incut_grid(C(ctx), F(fw), n, &v);
+        *out = v;
+        return st;
+    });
+}
+
+int er_count_incut_widom(er_context *ctx, er_framework *fw, uint64_t n_insertions, uint64_t seed,
+                         uint64_t *out) {
+    if (!ctx || !fw || !out) return arg_error("er_count_incut_widom: null argument");
+    return guarded([&]() -> int {
+        unsigned long long v = 0;
+        int st = er::count_incut_widom(C(ctx), F(fw), n_insertions, seed, &v);
+        *out = v;
+        return st;
+    });
+}
+
+void er_henry_result(double sum_b, double sum_eb, double n_total, double temperature,
+                     double density, double *kh_mol_per_kg_pa, double *kh_mol_per_m3_pa,
+                     double *e_avg_k, double *e_avg_kj_per_mol) {
+    // src/henry.cu:166-174
+    const double e_avg = sum_eb / sum_b;
+    const double kh = sum_b / n_total / temperature / 8.314;
+    if (e_avg_k) *e_avg_k = e_avg;
+    if (e_avg_kj_per_mol) *e_avg_kj_per_mol = e_avg * 8.314 / 1000.0;
+    if (kh_mol_per_m3_pa) *kh_mol_per_m3_pa = kh;
+    if (kh_mol_per_kg_pa) *kh_mol_per_kg_pa = kh / density;
+}
+
+int er_batch_create(er_context *ctx, const er_task *tasks, int n_tasks, int use_graph,
+                    er_batch **out) {
+    if (!ctx || !out || (n_tasks > 0 && !tasks) || n_tasks < 0)
+        return arg_error("er_batch_create: bad arguments");
+    return guarded([&]() -> int {
+        std::vector<er::Task> tk(n_tasks);
+        for (int i = 0; i < n_tasks; i++) {
+            tk[i].fw = F(tasks[i].fw);
+            for (int d = 0; d < 3; d++) tk[i].grid_n[d] = tasks[i].grid_n[d];
+            tk[i].precision = tasks[i].precision;
+            tk[i].n_ins = tasks[i].n_insertions;
+            tk[i].temperature = tasks[i].temperature;
+            tk[i].seed = tasks[i].seed;
+        }
+        er::Batch *b = nullptr;
+        int st = er::batch_create(C(ctx), tk.data(), n_tasks, use_graph != 0, &b);
+        if (st == ST_OK) *out = reinterpret_cast<er_batch *>(b);
+        return st;
+    });
+}
+
+int er_batch_run(er_batch *batch) {
+    if (!batch) return arg_error("er_batch_run: null batch");
+    return guarded([&]() -> int { return er::batch_run(B(batch)); });
+}
+
+int er_batch_wait(er_batch *batch) {
+    if (!batch) return arg_error("er_batch_wait: null batch");
+    return guarded([&]() -> int { return er::batch_wait(B(batch)); });
+}
+
+int er_batch_result(er_batch *batch, int task, er_widom_result *out, er_grid **grid_out) {
+    if (!batch) return arg_error("er_batch_result: null batch");
+    er::WidomResult r;
+    er::GridDev *g = nullptr;
+    int st = er::batch_result(B(batch), task, &r, &g);
+    if (st != ST_OK) return st;
+    if (out) {
+        out->sum_b = r.sum_b;
+        out->sum_eb = r.sum_eb;
+        out->n = r.n;
+    }
+    if (grid_out) *grid_out = reinterpret_cast<er_grid *>(g);
+    return ST_OK;
+}
+
+int er_batch_set_timing(er_batch *batch, int on) {
+    if (!batch) return arg_error("er_batch_set_timing: null batch");
+    return guarded([&]() -> int { return er::batch_set_timing(B(batch), on != 0); });
+}
+
+int er_batch_kernel_ms(er_batch *batch, double *ms, int *launches) {
+    if (!batch || !ms) return arg_error("er_batch_kernel_ms: null argument");
+    return guarded([&]() -> int { return er::batch_kernel_ms(B(batch), ms, launches); });
+}
+
+int er_batch_release(er_batch *batch) {
+    delete B(batch);
+    return ST_OK;
+}
+
+int er_host_setup(const char *root, const char *structure, const char *adsorbate, int mode,
+                  er_host_run **out) {
+    if (!structure || !adsorbate || !out || (mode != 0 && mode != 1))
+        return arg_error("er_host_setup: bad arguments");
+    return guarded([&]() -> int {
+        er_host_run *r = new er_host_run();
+        r->mode = mode;
+        std::string msg;
+        const std::string rt = root ? root : ".";
+        int st = mode == 0 ? er::host::setup_grid_run(rt, structure, adsorbate, &r->gp, &r->setup,
+                                                      &msg)
+                           : er::host::setup_henry_run(rt, structure, adsorbate, &r->hp, &r->setup,
+                                                       &msg);
+        if (st != ST_OK) {
+            er::set_error(msg);
+            delete r;
+            return st;
+        }
+        *out = r;
+        return ST_OK;
+    });
+}
+
+int er_host_run_info(const er_host_run *r, er_host_info *o) {
+    if (!r || !o) return arg_error("er_host_run_info: null argument");
+    std::memset(o, 0, sizeof(*o));
+    const er::host::Framework &fw = r->setup.fw;
+    o->a = fw.a;
+    o->b = fw.b;
+    o->c = fw.c;
+    o->alpha = fw.alpha;
+    o->beta = fw.beta;
+    o->gamma = fw.gamma;
+    o->n_atoms = fw.noatoms;
+    o->density = fw.density;
+    o->volume = fw.volume_unitcell;
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) {
+            o->t_matrix[3 * i + j] = fw.t_matrix[i][j];
+            o->inv_t_matrix[3 * i + j] = fw.inv_t_matrix[i][j];
+        }
+    if (r->mode == 0) {
+        const er::host::GridParams &p = r->gp;
+        for (int d = 0; d < 3; d++) o->rep[d] = p.rep[d];
+        o->r_cutoff_squared = p.r_cutoff_squared;
+        o->temperature = p.T;
+        o->epsilon_guest = p.epsilon_guest;
+        o->sigma_guest = p.sigma_guest;
+        o->adsorbate_mw = p.adsorbateMW;
+        o->grid_n[0] = p.N_x;
+        o->grid_n[1] = p.N_y;
+        o->grid_n[2] = p.N_z;
+        o->grid_resolution = p.grid_resolution;
+        o->grid_precision = p.precision;
+    } else {
+        const er::host::HenryParams &p = r->hp;
+        for (int d = 0; d < 3; d++) o->rep[d] = p.rep[d];
+        o->r_cutoff_squared = p.r_cutoff_squared;
+        o->temperature = p.T;
+        o->epsilon_guest = p.epsilon_guest;
+        o->sigma_guest = p.sigma_guest;
+        o->adsorbate_mw = p.adsorbateMW;
+        o->insertions_per_a3 = p.numinsertionsperA3;
+        o->numinsertions = p.numinsertions;
+        o->insertions_per_thread = p.numinsertionsperthread;
+        o->total_insertions = er::host::henry_total(p);
+        o->seed = p.seed;
+    }
+    return ST_OK;
+}
+
+int er_host_run_particles(const er_host_run *r, er_particle *out, int capacity) {
+    if (!r || !out) return arg_error("er_host_run_particles: null argument");
+    const int n = (int)r->setup.atoms.size();
+    if (capacity < n) return arg_error("er_host_run_particles: capacity too small");
+    std::memcpy(out, r->setup.atoms.data(), sizeof(er_particle) * n);
+    return ST_OK;
+}
+
+int er_host_run_label(const er_host_run *r, int i, char *out, int capacity) {
+    if (!r || !out || capacity < 1 || i < 0 || i >= r->setup.fw.noatoms)
+        return arg_error("er_host_run_label: bad arguments");
+    std::snprintf(out, (size_t)capacity, "%s", r->setup.fw.identity[i].c_str());
+    return ST_OK;
+}
+
+int er_host_run_write_settings(const er_host_run *r, const char *path) {
+    if (!r || !path) return arg_error("er_host_run_write_settings: null argument");
+    FILE *f = std::fopen(path, "w");
+    if (!f) {
+        er::set_error(std::string("cannot open ") + path);
+        return ST_IO;
+    }
+    if (r->mode == 0)
+        er::host::write_grid_settings(f, r->gp, r->setup.fw, r->setup.ff, r->setup.atoms.data());
+    else
+        er::host::write_henry_settings(f, r->hp, r->setup.fw, r->setup.ff, r->setup.atoms.data());
+    std::fclose(f);
+    return ST_OK;
+}
+
+void er_host_run_release(er_host_run *r) { delete r; }
+
+int er_write_grid_file(const char *path, const char *format, const double *e64, const float *e32,
+                       const int n[3], const double t_matrix[9], int threads) {
+    if (!path || !format || !n || (!e64 == !e32) || !t_matrix)
+        return arg_error("er_write_grid_file: bad arguments");
+    const std::string fm = format;
+    if (fm != "txt" && fm != "cube") return arg_error("Grid output format must be txt or cube\n");
+    FILE *f = std::fopen(path, "w");
+    if (!f) {
+        er::set_error(std::string("cannot open ") + path);
+        return ST_IO;
+    }
+    double t[3][3];
+    for (int i = 0; i < 3; i++)
+        for (int j = 0; j < 3; j++) t[i][j] = t_matrix[3 * i + j];
+    if (fm == "txt") er::host::write_txt_header(f, n);
+    else er::host::write_cube_header(f, n, t);
+    int st = er::host::write_grid_values(f, fm, e64, e32, n, threads);
+    std::fclose(f);
+    if (st != ST_OK) er::set_error("er_write_grid_file: write failed");
+    return st;
+}
+
+}  // extern "C"

@@ -1,0 +1,745 @@
+// er_kernels.hip -- gfx950 kernels for the guest-framework Lennard-Jones
+// energy grid (src/computegridsheet.h:30-99) and Widom insertion
+// (src/doHenry.h:29-135).
+//
+// Layout in HBM, per framework:
+//   atoms   : AtomDev[n]           fractional coordinates + type, grouped by type
+//   near    : double4[]            Cartesian FP64 image atoms T(f_a + n) that can
+//                                  lie within r_c of the unit cell, grouped by type,
+//                                  reference enumeration order inside a type
+//   types   : TypeParams[ntypes]   mixed sigma^2, 4 eps, FP64 switch radius
+// The tiled kernel gives each workgroup one tile (a box of grid points or one
+// Widom bin).  The workgroup streams the near list once, keeps the image atoms
+// within r_c + R_tile of the tile in LDS (split into a "far" list that needs no
+// cutoff test and no FP64, and a "full" list), then every lane evaluates its
+// point(s) against the LDS list with the atom broadcast from LDS and the pair
+// parameters in scalar registers.
+#include "er_kernels.h"
+
+#include <float.h>
+
+namespace er {
+
+namespace {
+
+__device__ __forceinline__ int uniform(int v) { return __builtin_amdgcn_readfirstlane(v); }
+
+__device__ __forceinline__ int lanes_below(uint64_t mask) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(mask >> 32),
+                                     __builtin_amdgcn_mbcnt_lo((uint32_t)mask, 0u));
+}
+
+__device__ __forceinline__ void frac_to_cart(const double *t, double fx, double fy, double fz,
+                                             double &x, double &y, double &z) {
+    x = t[0] * fx + t[1] * fy + t[2] * fz;
+    y = t[3] * fx + t[4] * fy + t[5] * fz;
+    z = t[6] * fx + t[7] * fy + t[8] * fz;
+}
+
+// ---------------------------------------------------------------------------
+// Near-image list
+// ---------------------------------------------------------------------------
+
+struct NearItem {
+    bool valid;
+    double gx, gy, gz;  // fractional position of the image atom
+};
+
+__device__ __forceinline__ NearItem near_item(const NearPlan &p, int blk, int tid) {
+    NearItem it;
+    it.valid = false;
+    int t = 0;
+    while (t + 1 < p.ntypes && p.type_block0[t + 1] <= blk) ++t;
+    const int a0 = p.type_atom0[t];
+    const int cnt = p.type_atom0[t + 1] - a0;
+    const long long q = (long long)(blk - p.type_block0[t]) * kWG + tid;
+    if (cnt <= 0 || q >= (long long)cnt * p.n_images) return it;
+    const int m = (int)(q / cnt);
+    const int a = a0 + (int)(q % cnt);
+    const int nb = 2 * p.rep[1] + 1, nc = 2 * p.rep[2] + 1;
+    const int ii = m / (nb * nc) - p.rep[0];
+    const int jj = (m / nc) % nb - p.rep[1];
+    const int kk = m % nc - p.rep[2];
+    const AtomDev at = p.atoms[a];
+    // src/computegridsheet.h:61-63
+    it.gx = at.fx + 1.0 * ii;
+    it.gy = at.fy + 1.0 * jj;
+    it.gz = at.fz + 1.0 * kk;
+    it.valid = it.gx >= -p.ext[0] && it.gx <= 1.0 + p.ext[0] && it.gy >= -p.ext[1] &&
+               it.gy <= 1.0 + p.ext[1] && it.gz >= -p.ext[2] && it.gz <= 1.0 + p.ext[2];
+    return it;
+}
+
+__global__ __launch_bounds__(kWG) void k_near_count(NearPlan p, uint32_t *block_count) {
+    __shared__ int s_w[kWG / 64];
+    const int tid = threadIdx.x;
+    NearItem it = near_item(p, blockIdx.x, tid);
+    uint64_t m = __ballot(it.valid);
+    if ((tid & 63) == 0) s_w[tid >> 6] = __popcll(m);
+    __syncthreads();
+    if (tid == 0) block_count[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
+}
+
+__global__ __launch_bounds__(kWG) void k_near_write(NearPlan p, const uint32_t *block_off,
+                                                    double4 *near) {
+    __shared__ int s_w[kWG / 64];
+    const int tid = threadIdx.x, wave = tid >> 6;
+    NearItem it = near_item(p, blockIdx.x, tid);
+    uint64_t m = __ballot(it.valid);
+    if ((tid & 63) == 0) s_w[wave] = __popcll(m);
+    __syncthreads();
+    if (it.valid) {
+        int off = block_off[blockIdx.x] + lanes_below(m);
+        for (int w = 0; w < wave; ++w) off += s_w[w];
+        double x, y, z;
+        frac_to_cart(p.t, it.gx, it.gy, it.gz, x, y, z);
+        near[off] = make_double4(x, y, z, 0.0);
+    }
+}
+
+// Single-workgroup exclusive scan of n uint32 values; out[n] = total.
+__device__ void block_exclusive_scan(const uint32_t *in, long long n, uint32_t *out,
+                                     uint32_t *out2) {
+    __shared__ uint32_t s_part[kWG];
+    __shared__ uint32_t s_carry;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    const int per = 8;
+    for (long long base = 0; base < n; base += (long long)kWG * per) {
+        uint32_t v[per];
+        uint32_t sum = 0;
+        for (int k = 0; k < per; ++k) {
+            long long i = base + (long long)tid * per + k;
+            v[k] = (i < n) ? in[i] : 0u;
+            sum += v[k];
+        }
+        s_part[tid] = sum;
+        __syncthreads();
+        // Hillis-Steele inclusive scan over the 256 partial sums
+        for (int d = 1; d < kWG; d <<= 1) {
+            uint32_t add = (tid >= d) ? s_part[tid - d] : 0u;
+            __syncthreads();
+            s_part[tid] += add;
+            __syncthreads();
+        }
+        uint32_t run = s_carry + s_part[tid] - sum;
+        for (int k = 0; k < per; ++k) {
+            long long i = base + (long long)tid * per + k;
+            if (i < n) {
+                out[i] = run;
+                if (out2) out2[i] = run;
+            }
+            run += v[k];
+        }
+        __syncthreads();
+        if (tid == kWG - 1) s_carry += s_part[kWG - 1];
+        __syncthreads();
+    }
+    if (tid == 0) {
+        out[n] = s_carry;
+        if (out2) out2[n] = s_carry;
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_near_offsets(NearPlan p, const uint32_t *block_count,
+                                                      uint32_t *block_off, int *type_begin) {
+    block_exclusive_scan(block_count, p.nblocks, block_off, nullptr);
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        for (int t = 0; t < p.ntypes; ++t) type_begin[t] = (int)block_off[p.type_block0[t]];
+        type_begin[p.ntypes] = (int)block_off[p.nblocks];
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_scan_u32(const uint32_t *count, long long n,
+                                                  uint32_t *start, uint32_t *cursor) {
+    block_exclusive_scan(count, n, start, cursor);
+}
+
+// ---------------------------------------------------------------------------
+// Widom binning
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ uint32_t widom_bin(const double f[3], const int nb[3]) {
+    int b[3];
+    for (int d = 0; d < 3; ++d) {
+        int v = (int)(f[d] * nb[d]);  // f in (0,1]
+        b[d] = v < nb[d] ? v : nb[d] - 1;
+    }
+    return (uint32_t)((b[0] * nb[1] + b[1]) * nb[2] + b[2]);
+}
+
+__global__ __launch_bounds__(kWG) void k_widom_count(uint64_t seed, uint32_t stream, uint64_t n,
+                                                     int nb0, int nb1, int nb2, uint32_t *count) {
+    const int nb[3] = {nb0, nb1, nb2};
+    for (uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kWG) {
+        double f[3];
+        widom_point(seed, stream, i, f);
+        atomicAdd(&count[widom_bin(f, nb)], 1u);
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_widom_scatter(uint64_t seed, uint32_t stream, uint64_t n,
+                                                       int nb0, int nb1, int nb2,
+                                                       uint32_t *cursor, uint32_t *sorted) {
+    const int nb[3] = {nb0, nb1, nb2};
+    for (uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x; i < n;
+         i += (uint64_t)gridDim.x * kWG) {
+        double f[3];
+        widom_point(seed, stream, i, f);
+        uint32_t pos = atomicAdd(&cursor[widom_bin(f, nb)], 1u);
+        sorted[pos] = (uint32_t)i;
+    }
+}
+
+__global__ void k_fill_u32(uint32_t *p, uint32_t v, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+__global__ void k_fill_u64(unsigned long long *p, unsigned long long v, long long n) {
+    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
+         i += (long long)gridDim.x * blockDim.x)
+        p[i] = v;
+}
+
+// ---------------------------------------------------------------------------
+// Tiled energy kernel
+// ---------------------------------------------------------------------------
+
+// Margin (Angstrom) applied to every list classification so that FP64
+// rounding of the tile radius and distances can never misclassify a pair.
+constexpr double kMargin = 1e-6;
+
+enum : int { CAT_NONE = 0, CAT_FAR = 1, CAT_FULL = 2 };
+
+struct Tile {
+    double cx, cy, cz, R;
+};
+
+__device__ __forceinline__ Tile tile_sphere(const double *t, const double lo[3],
+                                            const double hi[3]) {
+    double mid[3] = {0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]), 0.5 * (lo[2] + hi[2])};
+    Tile s;
+    frac_to_cart(t, mid[0], mid[1], mid[2], s.cx, s.cy, s.cz);
+    double r2 = 0.0;
+    for (int c = 0; c < 8; ++c) {
+        double d[3] = {((c & 4) ? hi[0] : lo[0]) - mid[0], ((c & 2) ? hi[1] : lo[1]) - mid[1],
+                       ((c & 1) ? hi[2] : lo[2]) - mid[2]};
+        double x, y, z;
+        frac_to_cart(t, d[0], d[1], d[2], x, y, z);
+        double q = x * x + y * y + z * z;
+        r2 = q > r2 ? q : r2;
+    }
+    s.R = sqrt(r2) * (1.0 + 1e-12) + kMargin;
+    return s;
+}
+
+// Stream the near list of job J from (cur_t, cur_pos) into the LDS list until
+// it is exhausted or the list is full.  Far records grow from index 0, full
+// records from index cap-1 downwards.  Records keep the near-list order, so
+// the list (and every sum over it) depends only on the tile.  Returns the
+// number of segments written to seg (5 ints each: type, far begin/end, full
+// begin/end).
+template <int MATH>
+__device__ int fill_list(const JobDev *__restrict__ J, const Tile &tl, int &cur_t, int &cur_pos,
+                         int cap, double4 *__restrict__ L, int *__restrict__ seg,
+                         int *__restrict__ s_cnt) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int ntypes = J->ntypes;
+    const double rc = J->rc;
+    int nfar = 0, nfull = 0, nseg = 0, far0 = 0, full0 = 0;
+    while (cur_t < ntypes) {
+        const int tend = uniform(J->type_begin[cur_t + 1]);
+        if (cur_pos >= tend) {
+            if (nfar > far0 || nfull > full0) {
+                if (tid == 0) {
+                    int *s = seg + 5 * nseg;
+                    s[0] = cur_t; s[1] = far0; s[2] = nfar; s[3] = full0; s[4] = nfull;
+                }
+                ++nseg;
+            }
+            far0 = nfar;
+            full0 = nfull;
+            ++cur_t;
+            if (cur_t < ntypes) cur_pos = uniform(J->type_begin[cur_t]);
+            continue;
+        }
+        if (nfar + nfull + kWG > cap || nseg >= kMaxSeg) break;
+        const int idx = cur_pos + tid;
+        int cat = CAT_NONE;
+        double4 A = make_double4(0.0, 0.0, 0.0, 0.0);
+        if (idx < tend) {
+            A = J->near[idx];
+            const double dx = A.x - tl.cx, dy = A.y - tl.cy, dz = A.z - tl.cz;
+            const double d = sqrt(dx * dx + dy * dy + dz * dz);
+            if (d - tl.R < rc + kMargin) {
+                cat = CAT_FULL;
+                if (MATH == MATH_MIXED) {
+                    const double rclose = J->types[cur_t].rclose;
+                    if (d + tl.R < rc - kMargin && d - tl.R > rclose + kMargin) cat = CAT_FAR;
+                } else if (MATH == MATH_COUNT) {
+                    if (d + tl.R < rc - kMargin) cat = CAT_FAR;
+                }
+            }
+        }
+        const uint64_t bf = __ballot(cat == CAT_FAR);
+        const uint64_t bu = __ballot(cat == CAT_FULL);
+        if (lane == 0) {
+            s_cnt[2 * wave] = __popcll(bf);
+            s_cnt[2 * wave + 1] = __popcll(bu);
+        }
+        __syncthreads();
+        int off_f = nfar, off_u = nfull, tot_f = 0, tot_u = 0;
+#pragma unroll
+        for (int w = 0; w < kWG / 64; ++w) {
+            const int cf = s_cnt[2 * w], cu = s_cnt[2 * w + 1];
+            if (w < wave) {
+                off_f += cf;
+                off_u += cu;
+            }
+            tot_f += cf;
+            tot_u += cu;
+        }
+        if (cat == CAT_FAR) L[off_f + lanes_below(bf)] = A;
+        if (cat == CAT_FULL) L[cap - 1 - (off_u + lanes_below(bu))] = A;
+        nfar = uniform(nfar + tot_f);
+        nfull = uniform(nfull + tot_u);
+        cur_pos += kWG;
+        __syncthreads();
+    }
+    if (nfar > far0 || nfull > full0) {
+        if (tid == 0) {
+            int *s = seg + 5 * nseg;
+            s[0] = cur_t; s[1] = far0; s[2] = nfar; s[3] = full0; s[4] = nfull;
+        }
+        ++nseg;
+    }
+    return nseg;
+}
+
+// Energies (or in-cutoff counts) of this lane's PPL points against the LDS
+// list.  Far records: FP64 displacement and r^2, FP32 reciprocal and LJ,
+// FP32 partials flushed into the FP64 sum every 4 records.  Full records:
+// FP64 r^2, the reference's clamp and strict cutoff test, FP64 LJ with a
+// Newton-refined reciprocal.
+template <int PPL, int MATH>
+__device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
+                                          const double4 *__restrict__ L, int cap,
+                                          const int *__restrict__ seg, int nseg,
+                                          const double (&px)[PPL], const double (&py)[PPL],
+                                          const double (&pz)[PPL], double (&acc)[PPL]) {
+    const double rc2 = J->rc2, min_r2 = J->min_r2;
+    for (int s = 0; s < nseg; ++s) {
+        const int t = uniform(seg[5 * s]);
+        const int fb = uniform(seg[5 * s + 1]), fe = uniform(seg[5 * s + 2]);
+        const int ub = uniform(seg[5 * s + 3]), ue = uniform(seg[5 * s + 4]);
+        const TypeParams P = J->types[t];
+        if (MATH == MATH_COUNT) {
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) acc[q] += (double)(fe - fb);
+        } else if (MATH == MATH_MIXED) {
+            const float sig2f = P.sig2f, eps4f = P.eps4f;
+            int r = fb;
+            for (; r + 4 <= fe; r += 4) {
+                float a32[PPL];
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) a32[q] = 0.0f;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double4 A = L[r + u];
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q) {
+                        const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
+                        const double r2 = dx * dx + dy * dy + dz * dz;
+                        const float inv = __builtin_amdgcn_rcpf((float)r2);
+                        const float s2 = sig2f * inv;
+                        const float s6 = s2 * s2 * s2;
+                        a32[q] = __builtin_fmaf(eps4f, __builtin_fmaf(s6, s6, -s6), a32[q]);
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) acc[q] += (double)a32[q];
+            }
+            for (; r < fe; ++r) {
+                const double4 A = L[r];
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) {
+                    const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    const float inv = __builtin_amdgcn_rcpf((float)r2);
+                    const float s2 = sig2f * inv;
+                    const float s6 = s2 * s2 * s2;
+                    acc[q] += (double)(eps4f * __builtin_fmaf(s6, s6, -s6));
+                }
+            }
+        }
+        const double sig2 = P.sig2, eps4 = P.eps4;
+        for (int k = ub; k < ue; ++k) {
+            const double4 A = L[cap - 1 - k];
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) {
+                const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
+                double r2 = dx * dx + dy * dy + dz * dz;
+                r2 = r2 > min_r2 ? r2 : min_r2;  // src/computegridsheet.h:77
+                const bool in = r2 < rc2;        // src/computegridsheet.h:79
+                if (MATH == MATH_COUNT) {
+                    acc[q] += in ? 1.0 : 0.0;
+                } else {
+                    double y = (double)__builtin_amdgcn_rcpf((float)r2);
+                    y = fma(fma(-r2, y, 1.0), y, y);
+                    if (MATH == MATH_FP64) y = fma(fma(-r2, y, 1.0), y, y);
+                    const double s2 = sig2 * y;
+                    const double s6 = s2 * s2 * s2;
+                    const double e = eps4 * fma(s6, s6, -s6);
+                    acc[q] += in ? e : 0.0;
+                }
+            }
+        }
+    }
+}
+
+template <int PPL, int MATH>
+__global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, int njobs,
+                                               long long total_items,
+                                               unsigned long long *__restrict__ counter,
+                                               int cap) {
+    extern __shared__ double4 s_list[];
+    __shared__ int s_seg[5 * kMaxSeg];
+    __shared__ int s_cnt[2 * (kWG / 64)];
+    __shared__ long long s_item;
+    const int tid = threadIdx.x;
+
+    for (;;) {
+        if (tid == 0) s_item = (long long)atomicAdd(counter, 1ull);
+        __syncthreads();
+        const long long item = s_item;
+        __syncthreads();
+        if (item >= total_items) break;
+        int jl = 0, jh = njobs - 1;
+        while (jl < jh) {
+            const int mid = (jl + jh + 1) >> 1;
+            if (jobs[mid].item_begin <= item) jl = mid;
+            else jh = mid - 1;
+        }
+        const JobDev *__restrict__ J = jobs + jl;
+        const long long local = item - J->item_begin;
+        double lo[3], hi[3];
+
+        if (J->kind == JOB_GRID) {
+            // ---- one tile of grid points, index k + j*nz + i*ny*nz ----
+            int i0[3], i1[3];
+            long long rem = local;
+            int tc[3];
+            tc[2] = (int)(rem % J->ntiles[2]);
+            rem /= J->ntiles[2];
+            tc[1] = (int)(rem % J->ntiles[1]);
+            tc[0] = (int)(rem / J->ntiles[1]);
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                i0[d] = tc[d] * J->tile[d];
+                const int e = i0[d] + J->tile[d];
+                i1[d] = (e < J->n[d] ? e : J->n[d]) - 1;
+                // src/writegrid.cu:122-127
+                lo[d] = 1.0 * i0[d] / (J->n[d] - 1);
+                hi[d] = 1.0 * i1[d] / (J->n[d] - 1);
+            }
+            const Tile tl = tile_sphere(J->t, lo, hi);
+            double px[PPL], py[PPL], pz[PPL], acc[PPL];
+            long long oidx[PPL];
+            bool ok[PPL];
+            const int t1 = J->tile[1], t2 = J->tile[2];
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) {
+                const int p = q * kWG + tid;
+                int gi = i0[0] + p / (t1 * t2);
+                int gj = i0[1] + (p / t2) % t1;
+                int gk = i0[2] + p % t2;
+                ok[q] = gi <= i1[0] && gj <= i1[1] && gk <= i1[2];
+                if (!ok[q]) {
+                    gi = i0[0];
+                    gj = i0[1];
+                    gk = i0[2];
+                }
+                frac_to_cart(J->t, 1.0 * gi / (J->n[0] - 1), 1.0 * gj / (J->n[1] - 1),
+                             1.0 * gk / (J->n[2] - 1), px[q], py[q], pz[q]);
+                oidx[q] = (long long)gk + (long long)gj * J->n[2] +
+                          (long long)gi * J->n[1] * J->n[2];
+                acc[q] = 0.0;
+            }
+            int cur_t = 0, cur_pos = uniform(J->type_begin[0]);
+            do {
+                const int nseg =
+                    fill_list<MATH>(J, tl, cur_t, cur_pos, cap, s_list, s_seg, s_cnt);
+                __syncthreads();
+                eval_list<PPL, MATH>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
+                __syncthreads();
+            } while (cur_t < J->ntypes);
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) {
+                if (!ok[q]) continue;
+                if (MATH == MATH_COUNT) {
+                    ((uint32_t *)J->out)[oidx[q]] = (uint32_t)acc[q];
+                } else if (J->mode == OUT_F64) {
+                    ((double *)J->out)[oidx[q]] = acc[q];
+                } else {
+                    const double v = acc[q] < (double)FLT_MAX ? acc[q] : (double)FLT_MAX;
+                    ((float *)J->out)[oidx[q]] = (float)v;
+                }
+            }
+        } else {
+            // ---- one Widom bin: every insertion whose point falls in it ----
+            const int nb0 = J->nbins[0], nb1 = J->nbins[1], nb2 = J->nbins[2];
+            const int b = (int)local;
+            const int bc[3] = {b / (nb1 * nb2), (b / nb2) % nb1, b % nb2};
+            const int nbv[3] = {nb0, nb1, nb2};
+#pragma unroll
+            for (int d = 0; d < 3; ++d) {
+                lo[d] = (double)bc[d] / nbv[d];
+                hi[d] = (double)(bc[d] + 1) / nbv[d];
+            }
+            const uint32_t s0 = J->bin_start[b], s1 = J->bin_start[b + 1];
+            if (s0 == s1) continue;
+            const Tile tl = tile_sphere(J->t, lo, hi);
+            int cur_t = 0, cur_pos = uniform(J->type_begin[0]);
+            bool first = true;
+            do {
+                const int nseg =
+                    fill_list<MATH>(J, tl, cur_t, cur_pos, cap, s_list, s_seg, s_cnt);
+                __syncthreads();
+                for (uint32_t c = s0; c < s1; c += (uint32_t)(PPL * kWG)) {
+                    double px[PPL], py[PPL], pz[PPL], acc[PPL];
+                    uint32_t ins[PPL];
+                    bool ok[PPL];
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q) {
+                        const uint32_t p = c + (uint32_t)(q * kWG + tid);
+                        ok[q] = p < s1;
+                        ins[q] = J->sorted[ok[q] ? p : s0];
+                        double f[3];
+                        widom_point(J->seed, J->stream, ins[q], f);
+                        frac_to_cart(J->t, f[0], f[1], f[2], px[q], py[q], pz[q]);
+                        acc[q] = 0.0;
+                    }
+                    eval_list<PPL, MATH>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q) {
+                        if (!ok[q]) continue;
+                        double *dst = J->e_out + ins[q];
+                        *dst = first ? acc[q] : *dst + acc[q];
+                    }
+                }
+                first = false;
+                __syncthreads();
+            } while (cur_t < J->ntypes);
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// Reductions
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double wave_sum(double v) {
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
+    return v;
+}
+
+// Block g sums the Boltzmann factors b = exp(-E/T) (src/doHenry.h:107) and
+// E*b over the index range [g*n/G, (g+1)*n/G) in a fixed order.
+__global__ __launch_bounds__(kWG) void k_widom_partials(const double *e, long long n,
+                                                        double temperature, double2 *partials) {
+    __shared__ double s_b[kWG / 64], s_eb[kWG / 64];
+    const long long g = blockIdx.x, G = gridDim.x;
+    const long long lo = g * n / G, hi = (g + 1) * n / G;
+    double sb = 0.0, seb = 0.0;
+    for (long long i = lo + threadIdx.x; i < hi; i += kWG) {
+        const double E = e[i];
+        const double bz = exp(-E / temperature);
+        sb += bz;
+        seb += (bz == 0.0) ? 0.0 : E * bz;
+    }
+    sb = wave_sum(sb);
+    seb = wave_sum(seb);
+    if ((threadIdx.x & 63) == 0) {
+        s_b[threadIdx.x >> 6] = sb;
+        s_eb[threadIdx.x >> 6] = seb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partials[g] = make_double2((s_b[0] + s_b[1]) + (s_b[2] + s_b[3]),
+                                   (s_eb[0] + s_eb[1]) + (s_eb[2] + s_eb[3]));
+}
+
+__global__ void k_widom_final(const double2 *partials, int nblk, double *out) {
+    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+    double sb = 0.0, seb = 0.0;
+    for (int g = 0; g < nblk; ++g) {
+        sb += partials[g].x;
+        seb += partials[g].y;
+    }
+    out[0] = sb;
+    out[1] = seb;
+}
+
+__global__ __launch_bounds__(kWG) void k_sum_u32(const uint32_t *v, long long n,
+                                                 unsigned long long *out) {
+    unsigned long long s = 0;
+    for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kWG)
+        s += v[i];
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+__global__ __launch_bounds__(kWG) void k_sum_f64_counts(const double *v, long long n,
+                                                        unsigned long long *out) {
+    unsigned long long s = 0;
+    for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kWG)
+        s += (unsigned long long)v[i];
+    for (int o = 32; o >= 1; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) atomicAdd(out, s);
+}
+
+int grid_blocks(long long n, int per_block, int cap_blocks) {
+    long long b = (n + per_block - 1) / per_block;
+    if (b < 1) b = 1;
+    if (b > cap_blocks) b = cap_blocks;
+    return (int)b;
+}
+
+template <int PPL, int MATH>
+hipError_t launch_tiles_t(const JobDev *jobs, int njobs, long long total, unsigned long long *ctr,
+                          int wgs, int cap, hipStream_t s) {
+    hipLaunchKernelGGL((k_tiles<PPL, MATH>), dim3(wgs), dim3(kWG), tiles_lds_bytes(cap), s, jobs,
+                       njobs, total, ctr, cap);
+    return hipGetLastError();
+}
+
+template <int PPL, int MATH>
+int occupancy_t(int cap) {
+    int n = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<PPL, MATH>, kWG,
+                                                     tiles_lds_bytes(cap)) != hipSuccess)
+        return 1;
+    return n > 0 ? n : 1;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Launchers
+// ---------------------------------------------------------------------------
+
+hipError_t launch_near_count(const NearPlan &p, uint32_t *block_count, hipStream_t s) {
+    if (p.nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_near_count, dim3(p.nblocks), dim3(kWG), 0, s, p, block_count);
+    return hipGetLastError();
+}
+
+hipError_t launch_near_offsets(const NearPlan &p, const uint32_t *block_count,
+                               uint32_t *block_off, int *type_begin, hipStream_t s) {
+    hipLaunchKernelGGL(k_near_offsets, dim3(1), dim3(kWG), 0, s, p, block_count, block_off,
+                       type_begin);
+    return hipGetLastError();
+}
+
+hipError_t launch_near_write(const NearPlan &p, const uint32_t *block_off, double4 *near,
+                             hipStream_t s) {
+    if (p.nblocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_near_write, dim3(p.nblocks), dim3(kWG), 0, s, p, block_off, near);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u32, dim3(grid_blocks(n, kWG, 4096)), dim3(kWG), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, long long n,
+                           hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_fill_u64, dim3(grid_blocks(n, kWG, 4096)), dim3(kWG), 0, s, p, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_widom_count(uint64_t seed, uint32_t stream, uint64_t n, const int nb[3],
+                              uint32_t *count, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_widom_count, dim3(grid_blocks((long long)n, kWG, 8192)), dim3(kWG), 0, s,
+                       seed, stream, n, nb[0], nb[1], nb[2], count);
+    return hipGetLastError();
+}
+
+hipError_t launch_scan_u32(const uint32_t *count, long long n, uint32_t *start, uint32_t *cursor,
+                           hipStream_t s) {
+    hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(kWG), 0, s, count, n, start, cursor);
+    return hipGetLastError();
+}
+
+hipError_t launch_widom_scatter(uint64_t seed, uint32_t stream, uint64_t n, const int nb[3],
+                                uint32_t *cursor, uint32_t *sorted, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    hipLaunchKernelGGL(k_widom_scatter, dim3(grid_blocks((long long)n, kWG, 8192)), dim3(kWG), 0,
+                       s, seed, stream, n, nb[0], nb[1], nb[2], cursor, sorted);
+    return hipGetLastError();
+}
+
+size_t tiles_lds_bytes(int cap) { return (size_t)cap * sizeof(double4); }
+
+hipError_t launch_tiles(int math, int ppl, const JobDev *jobs, int njobs, long long total,
+                        unsigned long long *ctr, int wgs, int cap, hipStream_t s) {
+    if (total <= 0) return hipSuccess;
+    if (cap < 2 * kWG || wgs <= 0) return hipErrorInvalidValue;
+#define ER_TILES(P, M) \
+    if (ppl == P && math == M) return launch_tiles_t<P, M>(jobs, njobs, total, ctr, wgs, cap, s);
+    ER_TILES(1, MATH_MIXED) ER_TILES(2, MATH_MIXED)
+    ER_TILES(1, MATH_FP64) ER_TILES(2, MATH_FP64)
+    ER_TILES(1, MATH_COUNT) ER_TILES(2, MATH_COUNT)
+#undef ER_TILES
+    return hipErrorInvalidValue;
+}
+
+int tiles_occupancy(int math, int ppl, int cap) {
+#define ER_OCC(P, M) \
+    if (ppl == P && math == M) return occupancy_t<P, M>(cap);
+    ER_OCC(1, MATH_MIXED) ER_OCC(2, MATH_MIXED)
+    ER_OCC(1, MATH_FP64) ER_OCC(2, MATH_FP64)
+    ER_OCC(1, MATH_COUNT) ER_OCC(2, MATH_COUNT)
+#undef ER_OCC
+    return 1;
+}
+
+hipError_t launch_widom_partials(const double *e, long long n, double temperature,
+                                 double2 *partials, hipStream_t s) {
+    hipLaunchKernelGGL(k_widom_partials, dim3(kReduceBlocks), dim3(kWG), 0, s, e, n, temperature,
+                       partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_widom_final(const double2 *partials, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_widom_final, dim3(1), dim3(64), 0, s, partials, kReduceBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_u32(const uint32_t *v, long long n, unsigned long long *out, hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_u32, dim3(grid_blocks(n, kWG, 2048)), dim3(kWG), 0, s, v, n, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_sum_f64_counts(const double *v, long long n, unsigned long long *out,
+                                 hipStream_t s) {
+    if (n <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_sum_f64_counts, dim3(grid_blocks(n, kWG, 2048)), dim3(kWG), 0, s, v, n,
+                       out);
+    return hipGetLastError();
+}
+
+}  // namespace er

@@ -1,0 +1,914 @@
+// er_runtime.cpp -- see er_runtime.h.
+#include "er_runtime.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <sstream>
+
+namespace er {
+
+namespace {
+thread_local std::string g_last_error;
+
+#define HIP_TRY(expr)                                                                      \
+    do {                                                                                   \
+        hipError_t e_ = (expr);                                                            \
+        if (e_ != hipSuccess) {                                                            \
+            std::ostringstream os_;                                                        \
+            os_ << "HIP error " << hipGetErrorName(e_) << " (" << hipGetErrorString(e_)    \
+                << ") at " << __FILE__ << ":" << __LINE__ << " in " << #expr;             \
+            set_error(os_.str());                                                          \
+            return ST_HIP;                                                                 \
+        }                                                                                  \
+    } while (0)
+
+void invert3(const double t[9], double inv[9]) {
+    const double c00 = t[4] * t[8] - t[5] * t[7];
+    const double c01 = t[5] * t[6] - t[3] * t[8];
+    const double c02 = t[3] * t[7] - t[4] * t[6];
+    const double det = t[0] * c00 + t[1] * c01 + t[2] * c02;
+    inv[0] = c00 / det;
+    inv[1] = (t[2] * t[7] - t[1] * t[8]) / det;
+    inv[2] = (t[1] * t[5] - t[2] * t[4]) / det;
+    inv[3] = c01 / det;
+    inv[4] = (t[0] * t[8] - t[2] * t[6]) / det;
+    inv[5] = (t[2] * t[3] - t[0] * t[5]) / det;
+    inv[6] = c02 / det;
+    inv[7] = (t[1] * t[6] - t[0] * t[7]) / det;
+    inv[8] = (t[0] * t[4] - t[1] * t[3]) / det;
+}
+
+double det3(const double t[9]) {
+    return t[0] * (t[4] * t[8] - t[5] * t[7]) - t[1] * (t[3] * t[8] - t[5] * t[6]) +
+           t[2] * (t[3] * t[7] - t[4] * t[6]);
+}
+
+// Perpendicular width of the cell along fractional axis d: 1/|row_d(T^-1)|.
+double perp_width(const double inv[9], int d) {
+    const double *r = inv + 3 * d;
+    return 1.0 / std::sqrt(r[0] * r[0] + r[1] * r[1] + r[2] * r[2]);
+}
+
+template <class T>
+int pool_alloc(Context *ctx, size_t count, T **out, std::vector<void *> *owned) {
+    void *p = ctx->pool.alloc(count * sizeof(T) > 0 ? count * sizeof(T) : 16);
+    if (!p) return ST_NOMEM;
+    *out = (T *)p;
+    if (owned) owned->push_back(p);
+    return ST_OK;
+}
+
+#define ST_TRY(expr)                    \
+    do {                                \
+        int s_ = (expr);                \
+        if (s_ != ST_OK) return s_;     \
+    } while (0)
+
+}  // namespace
+
+void set_error(const std::string &msg) { g_last_error = msg; }
+const char *last_error() { return g_last_error.c_str(); }
+
+// ---------------------------------------------------------------------------
+// Device pool
+// ---------------------------------------------------------------------------
+
+size_t DevicePool::round_size(size_t b) {
+    if (b <= (1u << 20)) {
+        size_t r = 256;
+        while (r < b) r <<= 1;
+        return r;
+    }
+    const size_t mb = 1u << 20;
+    return (b + mb - 1) / mb * mb;
+}
+
+void *DevicePool::alloc(size_t bytes) {
+    std::lock_guard<std::mutex> lk(mu_);
+    const size_t sz = round_size(bytes);
+    auto it = cache_.find(sz);
+    if (it != cache_.end() && !it->second.empty()) {
+        void *p = it->second.back();
+        it->second.pop_back();
+        live_[p] = sz;
+        in_use_ += sz;
+        return p;
+    }
+    if (ctx_->hbm_cap && reserved_ + sz > ctx_->hbm_cap) {
+        // give cached blocks back before refusing
+        for (auto &kv : cache_)
+            for (void *p : kv.second) {
+                (void)hipFree(p);
+                reserved_ -= kv.first;
+            }
+        cache_.clear();
+        if (reserved_ + sz > ctx_->hbm_cap) {
+            std::ostringstream os;
+            os << "HBM cap exceeded: need " << sz << " B with " << reserved_ << " of "
+               << ctx_->hbm_cap << " B reserved";
+            set_error(os.str());
+            return nullptr;
+        }
+    }
+    void *p = nullptr;
+    if (hipMalloc(&p, sz) != hipSuccess) {
+        (void)hipGetLastError();
+        std::ostringstream os;
+        os << "hipMalloc of " << sz << " B failed";
+        set_error(os.str());
+        return nullptr;
+    }
+    live_[p] = sz;
+    in_use_ += sz;
+    reserved_ += sz;
+    return p;
+}
+
+void DevicePool::free(void *p) {
+    if (!p) return;
+    std::lock_guard<std::mutex> lk(mu_);
+    auto it = live_.find(p);
+    if (it == live_.end()) return;
+    cache_[it->second].push_back(p);
+    in_use_ -= it->second;
+    live_.erase(it);
+}
+
+void DevicePool::trim() {
+    std::lock_guard<std::mutex> lk(mu_);
+    for (auto &kv : cache_)
+        for (void *p : kv.second) {
+            (void)hipFree(p);
+            reserved_ -= kv.first;
+        }
+    cache_.clear();
+}
+
+DevicePool::~DevicePool() {
+    for (auto &kv : cache_)
+        for (void *p : kv.second) (void)hipFree(p);
+    for (auto &kv : live_) (void)hipFree(kv.first);
+}
+
+// ---------------------------------------------------------------------------
+// Context
+// ---------------------------------------------------------------------------
+
+Context::~Context() {
+    if (stream) {
+        (void)hipStreamSynchronize(stream);
+        (void)hipStreamDestroy(stream);
+    }
+    pool.free(d_counter);
+    pool.free(d_partials);
+    pool.free(d_sums);
+}
+
+int context_create(int device, size_t hbm_cap, Context **out) {
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        (void)hipGetLastError();
+        set_error("no HIP device available");
+        return ST_NODEVICE;
+    }
+    if (device < 0 || device >= ndev) {
+        set_error("device index out of range");
+        return ST_ARG;
+    }
+    HIP_TRY(hipSetDevice(device));
+    hipDeviceProp_t prop;
+    HIP_TRY(hipGetDeviceProperties(&prop, device));
+    Context *c = new Context();
+    c->device = device;
+    c->hbm_cap = hbm_cap;
+    c->n_cu = prop.multiProcessorCount;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess) {
+        delete c;
+        set_error("hipStreamCreateWithFlags failed");
+        return ST_HIP;
+    }
+    if (pool_alloc(c, 1, &c->d_counter, nullptr) != ST_OK ||
+        pool_alloc(c, kReduceBlocks, &c->d_partials, nullptr) != ST_OK ||
+        pool_alloc(c, 2, &c->d_sums, nullptr) != ST_OK) {
+        delete c;
+        return ST_NOMEM;
+    }
+    *out = c;
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Framework upload
+// ---------------------------------------------------------------------------
+
+FrameworkDev::~FrameworkDev() {
+    if (ctx)
+        for (void *p : owned) ctx->pool.free(p);
+}
+
+int framework_upload(Context *ctx, const Particle *atoms, int n, const double t[9],
+                     const int rep[3], double rc2, FrameworkDev **out) {
+    if (!ctx || (!atoms && n > 0) || n < 0 || !t || !rep || !out) {
+        set_error("framework_upload: null argument or negative atom count");
+        return ST_ARG;
+    }
+    if (rep[0] < 0 || rep[1] < 0 || rep[2] < 0 || rep[0] > 64 || rep[1] > 64 || rep[2] > 64) {
+        set_error("framework_upload: replication factors must be in [0, 64]");
+        return ST_ARG;
+    }
+    if (!(rc2 > 0.0) || !std::isfinite(rc2)) {
+        set_error("framework_upload: rc2 must be positive and finite");
+        return ST_ARG;
+    }
+    const double det = det3(t);
+    if (!(std::fabs(det) > 0.0) || !std::isfinite(det)) {
+        set_error("framework_upload: singular cell matrix");
+        return ST_ARG;
+    }
+    for (int i = 0; i < n; i++) {
+        const Particle &p = atoms[i];
+        if (!std::isfinite(p.x_f) || !std::isfinite(p.y_f) || !std::isfinite(p.z_f) ||
+            !std::isfinite(p.eps) || !std::isfinite(p.sig2) || std::fabs(p.x_f) > 1e6 ||
+            std::fabs(p.y_f) > 1e6 || std::fabs(p.z_f) > 1e6) {
+            set_error("framework_upload: non-finite or out-of-range atom");
+            return ST_ARG;
+        }
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    FrameworkDev *fw = new FrameworkDev();
+    fw->ctx = ctx;
+    fw->n_atoms = n;
+    std::memcpy(fw->t, t, sizeof(fw->t));
+    invert3(t, fw->inv_t);
+    fw->rc2 = rc2;
+    fw->rc = std::sqrt(rc2);
+    fw->volume = std::fabs(det);
+    std::memcpy(fw->rep, rep, sizeof(fw->rep));
+    fw->n_images = (long long)(2 * rep[0] + 1) * (2 * rep[1] + 1) * (2 * rep[2] + 1);
+
+    // Pair parameters: one type per distinct (eps_ij, sig2_ij), first-seen order.
+    std::vector<int> type_of(n);
+    std::map<std::pair<uint64_t, uint64_t>, int> seen;
+    for (int i = 0; i < n; i++) {
+        uint64_t ke, ks;
+        std::memcpy(&ke, &atoms[i].eps, 8);
+        std::memcpy(&ks, &atoms[i].sig2, 8);
+        auto key = std::make_pair(ke, ks);
+        auto it = seen.find(key);
+        if (it == seen.end()) {
+            int id = (int)fw->types.size();
+            seen[key] = id;
+            TypeParams tp;
+            tp.sig2 = atoms[i].sig2;
+            tp.eps4 = 4.0 * atoms[i].eps;
+            tp.rclose = std::sqrt(std::max(atoms[i].sig2, 0.0) / ctx->tuning.s2_switch);
+            tp.sig2f = (float)tp.sig2;
+            tp.eps4f = (float)tp.eps4;
+            fw->types.push_back(tp);
+            type_of[i] = id;
+        } else {
+            type_of[i] = it->second;
+        }
+    }
+    const int nt = (int)fw->types.size();
+    fw->ntypes = nt;
+    std::vector<int> type_atom0(nt + 1, 0), type_block0(nt + 1, 0);
+    for (int i = 0; i < n; i++) type_atom0[type_of[i] + 1]++;
+    for (int k = 0; k < nt; k++) type_atom0[k + 1] += type_atom0[k];
+    std::vector<AtomDev> grouped(n > 0 ? n : 1);
+    {
+        std::vector<int> fill(nt, 0);
+        for (int i = 0; i < n; i++) {
+            int k = type_of[i];
+            AtomDev &a = grouped[type_atom0[k] + fill[k]++];
+            a.fx = atoms[i].x_f;
+            a.fy = atoms[i].y_f;
+            a.fz = atoms[i].z_f;
+            a.type = k;
+            a.pad = 0;
+        }
+    }
+    for (int k = 0; k < nt; k++) {
+        long long work = (long long)(type_atom0[k + 1] - type_atom0[k]) * fw->n_images;
+        long long blocks = (work + kWG - 1) / kWG;
+        if (type_block0[k] + blocks > (1LL << 30)) {
+            delete fw;
+            set_error("framework_upload: too many image atoms");
+            return ST_ARG;
+        }
+        type_block0[k + 1] = type_block0[k] + (int)blocks;
+    }
+    NearPlan plan;
+    plan.ntypes = nt;
+    std::memcpy(plan.rep, rep, sizeof(plan.rep));
+    std::memcpy(plan.t, t, sizeof(plan.t));
+    size_t cap_atoms = 1;
+    for (int d = 0; d < 3; d++) {
+        const double row = 1.0 / perp_width(fw->inv_t, d);
+        plan.ext[d] = fw->rc * row + 1e-7;
+        const long long per_axis =
+            std::min<long long>(2LL * rep[d] + 1, (long long)std::floor(1.0 + 2.0 * plan.ext[d]) + 1);
+        cap_atoms *= (size_t)std::max<long long>(per_axis, 1);
+    }
+    plan.nblocks = type_block0[nt];
+    plan.n_images = (int)fw->n_images;
+    fw->near_capacity = std::max<size_t>(1, (size_t)n * cap_atoms);
+    int st = ST_OK;
+    if ((st = pool_alloc(ctx, n > 0 ? n : 1, &fw->d_atoms, &fw->owned)) ||
+        (st = pool_alloc(ctx, nt + 1, &fw->d_type_atom0, &fw->owned)) ||
+        (st = pool_alloc(ctx, nt + 1, &fw->d_type_block0, &fw->owned)) ||
+        (st = pool_alloc(ctx, nt + 1, &fw->d_type_begin, &fw->owned)) ||
+        (st = pool_alloc(ctx, nt > 0 ? nt : 1, &fw->d_types, &fw->owned)) ||
+        (st = pool_alloc(ctx, fw->near_capacity, &fw->d_near, &fw->owned)) ||
+        (st = pool_alloc(ctx, plan.nblocks + 1, &fw->d_block_count, &fw->owned)) ||
+        (st = pool_alloc(ctx, plan.nblocks + 1, &fw->d_block_off, &fw->owned))) {
+        delete fw;
+        return st;
+    }
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(fw->d_atoms, grouped.data(), sizeof(AtomDev) * std::max(n, 1),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(fw->d_type_atom0, type_atom0.data(), sizeof(int) * (nt + 1),
+                           hipMemcpyHostToDevice, s));
+    HIP_TRY(hipMemcpyAsync(fw->d_type_block0, type_block0.data(), sizeof(int) * (nt + 1),
+                           hipMemcpyHostToDevice, s));
+    if (nt > 0)
+        HIP_TRY(hipMemcpyAsync(fw->d_types, fw->types.data(), sizeof(TypeParams) * nt,
+                               hipMemcpyHostToDevice, s));
+    plan.atoms = fw->d_atoms;
+    plan.type_atom0 = fw->d_type_atom0;
+    plan.type_block0 = fw->d_type_block0;
+    HIP_TRY(launch_near_count(plan, fw->d_block_count, s));
+    HIP_TRY(launch_near_offsets(plan, fw->d_block_count, fw->d_block_off, fw->d_type_begin, s));
+    HIP_TRY(launch_near_write(plan, fw->d_block_off, fw->d_near, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    *out = fw;
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Job construction
+// ---------------------------------------------------------------------------
+
+void choose_grid_tile(const FrameworkDev *fw, const int n[3], const Tuning &tu, int tile[3]) {
+    (void)fw;
+    (void)n;
+    tile[0] = tu.tile[0];
+    tile[1] = tu.tile[1];
+    tile[2] = tu.tile[2];
+}
+
+void choose_widom_bins(const FrameworkDev *fw, uint64_t n_ins, const Tuning &tu, int nb[3]) {
+    double w[3];
+    for (int d = 0; d < 3; d++) w[d] = perp_width(fw->inv_t, d);
+    double target = (double)n_ins / std::max(tu.widom_per_bin, 1);
+    target = std::min(std::max(target, 1.0), (double)(1 << 21));
+    const double k = std::cbrt(target / (w[0] * w[1] * w[2]));
+    for (int d = 0; d < 3; d++) nb[d] = std::max(1, std::min(1 << 10, (int)std::lround(w[d] * k)));
+}
+
+JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void *d_out,
+                     const Tuning &tu, int math) {
+    JobDev j;
+    std::memset(&j, 0, sizeof(j));
+    std::memcpy(j.t, fw->t, sizeof(j.t));
+    j.rc2 = fw->rc2;
+    j.rc = fw->rc;
+    j.min_r2 = 1e-12 * 1e-12;  // src/computegridsheet.h:19,77
+    j.near = fw->d_near;
+    j.type_begin = fw->d_type_begin;
+    j.types = fw->d_types;
+    j.ntypes = fw->ntypes;
+    j.kind = JOB_GRID;
+    j.mode = math == MATH_COUNT ? OUT_COUNT : (precision == 1 ? OUT_F64 : OUT_F32);
+    int tile[3];
+    choose_grid_tile(fw, n, tu, tile);
+    long long items = 1;
+    for (int d = 0; d < 3; d++) {
+        j.n[d] = n[d];
+        j.tile[d] = tile[d];
+        j.ntiles[d] = (n[d] + tile[d] - 1) / tile[d];
+        items *= j.ntiles[d];
+    }
+    j.out = d_out;
+    j.n_items = items;
+    return j;
+}
+
+JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
+                      const WidomBuffers &wb) {
+    JobDev j;
+    std::memset(&j, 0, sizeof(j));
+    std::memcpy(j.t, fw->t, sizeof(j.t));
+    j.rc2 = fw->rc2;
+    j.rc = fw->rc;
+    j.min_r2 = 1e-16 * 1e-16;  // src/doHenry.h:15,91
+    j.near = fw->d_near;
+    j.type_begin = fw->d_type_begin;
+    j.types = fw->d_types;
+    j.ntypes = fw->ntypes;
+    j.kind = JOB_WIDOM;
+    j.mode = OUT_F64;
+    for (int d = 0; d < 3; d++) j.nbins[d] = wb.nbins[d];
+    j.sorted = wb.sorted;
+    j.bin_start = wb.start;
+    j.seed = seed;
+    j.stream = 0;
+    j.e_out = wb.e;
+    j.n_items = (long long)wb.nbins[0] * wb.nbins[1] * wb.nbins[2];
+    (void)n_ins;
+    return j;
+}
+
+namespace {
+
+int check_grid_dims(const int n[3]) {
+    for (int d = 0; d < 3; d++)
+        if (n[d] < 2 || n[d] > 65536) {
+            set_error("grid dimensions must be in [2, 65536] (the reference divides by N-1)");
+            return ST_ARG;
+        }
+    const long long p = (long long)n[0] * n[1] * n[2];
+    if (p > (1LL << 36)) {
+        set_error("grid too large");
+        return ST_ARG;
+    }
+    return ST_OK;
+}
+
+int check_tuning(const Tuning &tu) {
+    if (tu.ppl != 1 && tu.ppl != 2) {
+        set_error("tuning: points per lane must be 1 or 2");
+        return ST_ARG;
+    }
+    if (tu.tile[0] * tu.tile[1] * tu.tile[2] != kWG * tu.ppl || tu.tile[0] < 1 || tu.tile[1] < 1 ||
+        tu.tile[2] < 1) {
+        set_error("tuning: tile extents must multiply to 256 * points-per-lane");
+        return ST_ARG;
+    }
+    if (tu.cap < 2 * kWG || tu.cap > 4800) {
+        set_error("tuning: LDS list capacity must be in [512, 4800] records");
+        return ST_ARG;
+    }
+    return ST_OK;
+}
+
+int persistent_wgs(Context *ctx, int math, long long items) {
+    const int occ = tiles_occupancy(math, ctx->tuning.ppl, ctx->tuning.cap);
+    long long w = (long long)ctx->n_cu * occ;
+    if (w > items) w = items;
+    return (int)std::max<long long>(w, 1);
+}
+
+// Launch one job array on the context stream with the context's counter.
+int run_jobs(Context *ctx, int math, JobDev *d_jobs, int njobs, long long total,
+             unsigned long long *d_counter) {
+    hipStream_t s = ctx->stream;
+    HIP_TRY(launch_fill_u64(d_counter, 0ull, 1, s));
+    HIP_TRY(launch_tiles(math, ctx->tuning.ppl, d_jobs, njobs, total, d_counter,
+                         persistent_wgs(ctx, math, total), ctx->tuning.cap, s));
+    return ST_OK;
+}
+
+int widom_prepare(Context *ctx, FrameworkDev *fw, uint64_t n_ins, WidomBuffers *wb,
+                  std::vector<void *> *owned) {
+    if (n_ins == 0 || n_ins >= (1ULL << 32)) {
+        set_error("Widom insertion count must be in [1, 2^32)");
+        return ST_ARG;
+    }
+    choose_widom_bins(fw, n_ins, ctx->tuning, wb->nbins);
+    const long long nb = (long long)wb->nbins[0] * wb->nbins[1] * wb->nbins[2];
+    ST_TRY(pool_alloc(ctx, nb + 1, &wb->count, owned));
+    ST_TRY(pool_alloc(ctx, nb + 1, &wb->start, owned));
+    ST_TRY(pool_alloc(ctx, nb + 1, &wb->cursor, owned));
+    ST_TRY(pool_alloc(ctx, n_ins, &wb->sorted, owned));
+    ST_TRY(pool_alloc(ctx, n_ins, &wb->e, owned));
+    return ST_OK;
+}
+
+int widom_enqueue_binning(Context *ctx, const WidomBuffers &wb, uint64_t n_ins, uint64_t seed,
+                          uint32_t stream_id, hipStream_t s) {
+    const long long nb = (long long)wb.nbins[0] * wb.nbins[1] * wb.nbins[2];
+    (void)ctx;
+    HIP_TRY(launch_fill_u32(wb.count, 0u, nb + 1, s));
+    HIP_TRY(launch_widom_count(seed, stream_id, n_ins, wb.nbins, wb.count, s));
+    HIP_TRY(launch_scan_u32(wb.count, nb, wb.start, wb.cursor, s));
+    HIP_TRY(launch_widom_scatter(seed, stream_id, n_ins, wb.nbins, wb.cursor, wb.sorted, s));
+    return ST_OK;
+}
+
+struct Scoped {
+    Context *ctx;
+    std::vector<void *> ptrs;
+    ~Scoped() {
+        for (void *p : ptrs) ctx->pool.free(p);
+    }
+};
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// Plain path
+// ---------------------------------------------------------------------------
+
+size_t GridDev::bytes() const {
+    return (size_t)n[0] * n[1] * n[2] * (precision == 1 ? sizeof(double) : sizeof(float));
+}
+
+GridDev::~GridDev() {
+    if (ctx) ctx->pool.free(d_data);
+}
+
+int grid_compute(Context *ctx, FrameworkDev *fw, const int n[3], int precision, void *host_out,
+                 GridDev **resident) {
+    if (!ctx || !fw || !n) {
+        set_error("grid_compute: null argument");
+        return ST_ARG;
+    }
+    ST_TRY(check_grid_dims(n));
+    ST_TRY(check_tuning(ctx->tuning));
+    if (precision != 0 && precision != 1) {
+        set_error("grid_compute: precision must be 0 (FP32) or 1 (FP64)");
+        return ST_ARG;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    GridDev *g = new GridDev();
+    g->ctx = ctx;
+    g->fw = fw;
+    std::memcpy(g->n, n, sizeof(g->n));
+    g->precision = precision;
+    g->d_data = ctx->pool.alloc(g->bytes());
+    if (!g->d_data) {
+        delete g;
+        return ST_NOMEM;
+    }
+    Scoped tmp{ctx, {}};
+    const int math = precision == 1 ? MATH_FP64 : MATH_MIXED;
+    JobDev job = make_grid_job(fw, n, precision, g->d_data, ctx->tuning, math);
+    JobDev *d_job = nullptr;
+    int st = pool_alloc(ctx, 1, &d_job, &tmp.ptrs);
+    if (st) {
+        delete g;
+        return st;
+    }
+    hipStream_t s = ctx->stream;
+    auto fail = [&](int code) {
+        delete g;
+        return code;
+    };
+    if (hipMemcpyAsync(d_job, &job, sizeof(job), hipMemcpyHostToDevice, s) != hipSuccess)
+        return fail(ST_HIP);
+    if ((st = run_jobs(ctx, math, d_job, 1, job.n_items, ctx->d_counter))) return fail(st);
+    if (host_out &&
+        hipMemcpyAsync(host_out, g->d_data, g->bytes(), hipMemcpyDeviceToHost, s) != hipSuccess) {
+        set_error("grid_compute: device-to-host copy failed");
+        return fail(ST_HIP);
+    }
+    if (hipStreamSynchronize(s) != hipSuccess) {
+        set_error(std::string("grid_compute: ") + hipGetErrorString(hipGetLastError()));
+        return fail(ST_HIP);
+    }
+    if (resident) *resident = g;
+    else delete g;
+    return ST_OK;
+}
+
+int widom_run(Context *ctx, FrameworkDev *fw, uint64_t n_ins, double temperature, uint64_t seed,
+              WidomResult *out, double *energies_host) {
+    if (!ctx || !fw || !out) {
+        set_error("widom: null argument");
+        return ST_ARG;
+    }
+    if (!(temperature > 0.0) || !std::isfinite(temperature)) {
+        set_error("widom: temperature must be positive");
+        return ST_ARG;
+    }
+    ST_TRY(check_tuning(ctx->tuning));
+    HIP_TRY(hipSetDevice(ctx->device));
+    Scoped tmp{ctx, {}};
+    WidomBuffers wb;
+    ST_TRY(widom_prepare(ctx, fw, n_ins, &wb, &tmp.ptrs));
+    JobDev job = make_widom_job(fw, n_ins, seed, wb);
+    JobDev *d_job = nullptr;
+    ST_TRY(pool_alloc(ctx, 1, &d_job, &tmp.ptrs));
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(d_job, &job, sizeof(job), hipMemcpyHostToDevice, s));
+    ST_TRY(widom_enqueue_binning(ctx, wb, n_ins, seed, 0u, s));
+    ST_TRY(run_jobs(ctx, MATH_MIXED, d_job, 1, job.n_items, ctx->d_counter));
+    HIP_TRY(launch_widom_partials(wb.e, (long long)n_ins, temperature, ctx->d_partials, s));
+    HIP_TRY(launch_widom_final(ctx->d_partials, ctx->d_sums, s));
+    double sums[2];
+    HIP_TRY(hipMemcpyAsync(sums, ctx->d_sums, sizeof(sums), hipMemcpyDeviceToHost, s));
+    if (energies_host)
+        HIP_TRY(hipMemcpyAsync(energies_host, wb.e, sizeof(double) * n_ins, hipMemcpyDeviceToHost,
+                               s));
+    HIP_TRY(hipStreamSynchronize(s));
+    out->sum_b = sums[0];
+    out->sum_eb = sums[1];
+    out->n = n_ins;
+    return ST_OK;
+}
+
+int count_incut_grid(Context *ctx, FrameworkDev *fw, const int n[3], unsigned long long *out) {
+    ST_TRY(check_grid_dims(n));
+    ST_TRY(check_tuning(ctx->tuning));
+    HIP_TRY(hipSetDevice(ctx->device));
+    Scoped tmp{ctx, {}};
+    uint32_t *d_cnt = nullptr;
+    unsigned long long *d_sum = nullptr;
+    JobDev *d_job = nullptr;
+    const long long p = (long long)n[0] * n[1] * n[2];
+    ST_TRY(pool_alloc(ctx, p, &d_cnt, &tmp.ptrs));
+    ST_TRY(pool_alloc(ctx, 1, &d_sum, &tmp.ptrs));
+    ST_TRY(pool_alloc(ctx, 1, &d_job, &tmp.ptrs));
+    JobDev job = make_grid_job(fw, n, 0, d_cnt, ctx->tuning, MATH_COUNT);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(d_job, &job, sizeof(job), hipMemcpyHostToDevice, s));
+    ST_TRY(run_jobs(ctx, MATH_COUNT, d_job, 1, job.n_items, ctx->d_counter));
+    HIP_TRY(launch_fill_u64(d_sum, 0ull, 1, s));
+    HIP_TRY(launch_sum_u32(d_cnt, p, d_sum, s));
+    HIP_TRY(hipMemcpyAsync(out, d_sum, sizeof(*out), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ST_OK;
+}
+
+int count_incut_widom(Context *ctx, FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
+                      unsigned long long *out) {
+    ST_TRY(check_tuning(ctx->tuning));
+    HIP_TRY(hipSetDevice(ctx->device));
+    Scoped tmp{ctx, {}};
+    WidomBuffers wb;
+    ST_TRY(widom_prepare(ctx, fw, n_ins, &wb, &tmp.ptrs));
+    unsigned long long *d_sum = nullptr;
+    JobDev *d_job = nullptr;
+    ST_TRY(pool_alloc(ctx, 1, &d_sum, &tmp.ptrs));
+    ST_TRY(pool_alloc(ctx, 1, &d_job, &tmp.ptrs));
+    JobDev job = make_widom_job(fw, n_ins, seed, wb);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(d_job, &job, sizeof(job), hipMemcpyHostToDevice, s));
+    ST_TRY(widom_enqueue_binning(ctx, wb, n_ins, seed, 0u, s));
+    ST_TRY(run_jobs(ctx, MATH_COUNT, d_job, 1, job.n_items, ctx->d_counter));
+    HIP_TRY(launch_fill_u64(d_sum, 0ull, 1, s));
+    HIP_TRY(launch_sum_f64_counts(wb.e, (long long)n_ins, d_sum, s));
+    HIP_TRY(hipMemcpyAsync(out, d_sum, sizeof(*out), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ST_OK;
+}
+
+// ---------------------------------------------------------------------------
+// Batched path
+// ---------------------------------------------------------------------------
+
+Batch::~Batch() {
+    for (hipEvent_t e : ev)
+        if (e) (void)hipEventDestroy(e);
+    if (exec) (void)hipGraphExecDestroy(exec);
+    if (graph) (void)hipGraphDestroy(graph);
+    for (GridDev *g : grids) delete g;
+    if (h_sums) (void)hipHostFree(h_sums);
+    if (ctx)
+        for (void *p : owned) ctx->pool.free(p);
+}
+
+namespace {
+
+// Expected in-cutoff pairs per point: the ordering key of the LPT schedule.
+double job_cost_per_item(const FrameworkDev *fw, const JobDev &j, const Tuning &tu) {
+    const double pi = 3.14159265358979323846;
+    const double dens = fw->n_atoms / std::max(fw->volume, 1e-9);
+    double pts;
+    if (j.kind == JOB_GRID) pts = (double)j.tile[0] * j.tile[1] * j.tile[2];
+    else pts = (double)tu.widom_per_bin;
+    return pts * dens * (4.0 / 3.0) * pi * fw->rc2 * fw->rc;
+}
+
+}  // namespace
+
+int Batch::record(hipStream_t s) {
+    for (size_t t = 0; t < tasks.size(); t++)
+        if (tasks[t].n_ins > 0)
+            ST_TRY(widom_enqueue_binning(ctx, wbufs[t], tasks[t].n_ins, tasks[t].seed, 0u, s));
+    // jobs are partitioned: [0, n_mixed) mixed math, [n_mixed, njobs) FP64 grids
+    int n_mixed = 0;
+    long long items_mixed = 0, items_fp64 = 0;
+    for (const JobDev &j : jobs) {
+        if (j.kind == JOB_GRID && j.mode == OUT_F64) items_fp64 += j.n_items;
+        else {
+            n_mixed++;
+            items_mixed += j.n_items;
+        }
+    }
+    const bool ev_on = timing && !use_graph;
+    timed_launches = 0;
+    HIP_TRY(launch_fill_u64(d_counter, 0ull, 2, s));
+    if (items_mixed > 0) {
+        if (ev_on) HIP_TRY(hipEventRecord(ev[0], s));
+        HIP_TRY(launch_tiles(MATH_MIXED, ctx->tuning.ppl, d_jobs, n_mixed, items_mixed, d_counter,
+                             persistent_wgs(ctx, MATH_MIXED, items_mixed), ctx->tuning.cap, s));
+        if (ev_on) HIP_TRY(hipEventRecord(ev[1], s));
+        timed_launches |= 1;
+    }
+    if (items_fp64 > 0) {
+        if (ev_on) HIP_TRY(hipEventRecord(ev[2], s));
+        HIP_TRY(launch_tiles(MATH_FP64, ctx->tuning.ppl, d_jobs + n_mixed,
+                             (int)jobs.size() - n_mixed, items_fp64, d_counter + 1,
+                             persistent_wgs(ctx, MATH_FP64, items_fp64), ctx->tuning.cap, s));
+        if (ev_on) HIP_TRY(hipEventRecord(ev[3], s));
+        timed_launches |= 2;
+    }
+    for (size_t t = 0; t < tasks.size(); t++)
+        if (tasks[t].n_ins > 0) {
+            HIP_TRY(launch_widom_partials(wbufs[t].e, (long long)tasks[t].n_ins,
+                                          tasks[t].temperature, d_partials + t * kReduceBlocks, s));
+            HIP_TRY(launch_widom_final(d_partials + t * kReduceBlocks, d_sums + 2 * t, s));
+        }
+    return ST_OK;
+}
+
+int batch_create(Context *ctx, const Task *tasks, int n, bool use_graph, Batch **out) {
+    if (!ctx || (!tasks && n > 0) || n < 0 || !out) {
+        set_error("batch_create: bad arguments");
+        return ST_ARG;
+    }
+    ST_TRY(check_tuning(ctx->tuning));
+    HIP_TRY(hipSetDevice(ctx->device));
+    Batch *b = new Batch();
+    b->ctx = ctx;
+    b->use_graph = use_graph;
+    b->tasks.assign(tasks, tasks + n);
+    b->grids.assign(n, nullptr);
+    b->wbufs.assign(n, WidomBuffers());
+    auto fail = [&](int st) {
+        delete b;
+        return st;
+    };
+    std::vector<std::pair<double, JobDev>> mixed, fp64;
+    for (int t = 0; t < n; t++) {
+        const Task &tk = tasks[t];
+        if (!tk.fw) {
+            set_error("batch_create: task without framework");
+            return fail(ST_ARG);
+        }
+        const bool has_grid = tk.grid_n[0] || tk.grid_n[1] || tk.grid_n[2];
+        if (has_grid) {
+            int st = check_grid_dims(tk.grid_n);
+            if (st) return fail(st);
+            if (tk.precision != 0 && tk.precision != 1) {
+                set_error("batch_create: precision must be 0 or 1");
+                return fail(ST_ARG);
+            }
+            GridDev *g = new GridDev();
+            g->ctx = ctx;
+            g->fw = tk.fw;
+            std::memcpy(g->n, tk.grid_n, sizeof(g->n));
+            g->precision = tk.precision;
+            g->d_data = ctx->pool.alloc(g->bytes());
+            if (!g->d_data) {
+                delete g;
+                return fail(ST_NOMEM);
+            }
+            b->grids[t] = g;
+            const int math = tk.precision == 1 ? MATH_FP64 : MATH_MIXED;
+            JobDev j = make_grid_job(tk.fw, tk.grid_n, tk.precision, g->d_data, ctx->tuning, math);
+            (math == MATH_FP64 ? fp64 : mixed)
+                .push_back({job_cost_per_item(tk.fw, j, ctx->tuning), j});
+        }
+        if (tk.n_ins > 0) {
+            if (!(tk.temperature > 0.0)) {
+                set_error("batch_create: temperature must be positive");
+                return fail(ST_ARG);
+            }
+            int st = widom_prepare(ctx, tk.fw, tk.n_ins, &b->wbufs[t], &b->owned);
+            if (st) return fail(st);
+            JobDev j = make_widom_job(tk.fw, tk.n_ins, tk.seed, b->wbufs[t]);
+            mixed.push_back({job_cost_per_item(tk.fw, j, ctx->tuning), j});
+        }
+    }
+    // Longest-processing-time order: the most expensive items are handed out first.
+    auto by_cost = [](const std::pair<double, JobDev> &a, const std::pair<double, JobDev> &c) {
+        return a.first > c.first;
+    };
+    std::stable_sort(mixed.begin(), mixed.end(), by_cost);
+    std::stable_sort(fp64.begin(), fp64.end(), by_cost);
+    long long begin = 0;
+    for (auto &pj : mixed) {
+        pj.second.item_begin = begin;
+        begin += pj.second.n_items;
+        b->jobs.push_back(pj.second);
+    }
+    begin = 0;
+    for (auto &pj : fp64) {
+        pj.second.item_begin = begin;
+        begin += pj.second.n_items;
+        b->jobs.push_back(pj.second);
+    }
+    int st = ST_OK;
+    if ((st = pool_alloc(ctx, std::max<size_t>(b->jobs.size(), 1), &b->d_jobs, &b->owned)) ||
+        (st = pool_alloc(ctx, (size_t)std::max(n, 1) * kReduceBlocks, &b->d_partials, &b->owned)) ||
+        (st = pool_alloc(ctx, (size_t)std::max(n, 1) * 2, &b->d_sums, &b->owned)) ||
+        (st = pool_alloc(ctx, 2, &b->d_counter, &b->owned)))
+        return fail(st);
+    if (hipHostMalloc((void **)&b->h_sums, sizeof(double) * 2 * std::max(n, 1)) != hipSuccess) {
+        set_error("batch_create: pinned host allocation failed");
+        return fail(ST_NOMEM);
+    }
+    if (!b->jobs.empty() &&
+        hipMemcpyAsync(b->d_jobs, b->jobs.data(), sizeof(JobDev) * b->jobs.size(),
+                       hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+        set_error("batch_create: job upload failed");
+        return fail(ST_HIP);
+    }
+    if (hipStreamSynchronize(ctx->stream) != hipSuccess) {
+        set_error("batch_create: stream synchronize failed");
+        return fail(ST_HIP);
+    }
+    *out = b;
+    return ST_OK;
+}
+
+int batch_run(Batch *b) {
+    if (!b) {
+        set_error("batch_run: null batch");
+        return ST_ARG;
+    }
+    Context *ctx = b->ctx;
+    HIP_TRY(hipSetDevice(ctx->device));
+    if (!b->use_graph) return b->record(ctx->stream);
+    if (!b->exec) {
+        HIP_TRY(hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal));
+        int st = b->record(ctx->stream);
+        hipGraph_t g = nullptr;
+        hipError_t e = hipStreamEndCapture(ctx->stream, &g);
+        if (st != ST_OK) {
+            if (g) (void)hipGraphDestroy(g);
+            return st;
+        }
+        HIP_TRY(e);
+        b->graph = g;
+        HIP_TRY(hipGraphInstantiate(&b->exec, b->graph, nullptr, nullptr, 0));
+    }
+    HIP_TRY(hipGraphLaunch(b->exec, ctx->stream));
+    return ST_OK;
+}
+
+int batch_wait(Batch *b) {
+    if (!b) {
+        set_error("batch_wait: null batch");
+        return ST_ARG;
+    }
+    Context *ctx = b->ctx;
+    const size_t n = b->tasks.size();
+    if (n > 0)
+        HIP_TRY(hipMemcpyAsync(b->h_sums, b->d_sums, sizeof(double) * 2 * n,
+                               hipMemcpyDeviceToHost, ctx->stream));
+    HIP_TRY(hipStreamSynchronize(ctx->stream));
+    return ST_OK;
+}
+
+int batch_set_timing(Batch *b, bool on) {
+    if (!b) {
+        set_error("batch_set_timing: null batch");
+        return ST_ARG;
+    }
+    if (on && !b->ev[0])
+        for (hipEvent_t &e : b->ev) HIP_TRY(hipEventCreate(&e));
+    b->timing = on;
+    return ST_OK;
+}
+
+int batch_kernel_ms(Batch *b, double *ms, int *launches) {
+    if (!b || !ms) {
+        set_error("batch_kernel_ms: null argument");
+        return ST_ARG;
+    }
+    *ms = 0.0;
+    int n = 0;
+    if (b->timing && !b->use_graph) {
+        for (int k = 0; k < 2; k++)
+            if (b->timed_launches & (1 << k)) {
+                float t = 0.0f;
+                HIP_TRY(hipEventElapsedTime(&t, b->ev[2 * k], b->ev[2 * k + 1]));
+                *ms += t;
+                n++;
+            }
+    }
+    if (launches) *launches = n;
+    return ST_OK;
+}
+
+int batch_result(Batch *b, int i, WidomResult *out, GridDev **grid) {
+    if (!b || i < 0 || i >= (int)b->tasks.size()) {
+        set_error("batch_result: index out of range");
+        return ST_ARG;
+    }
+    if (out) {
+        out->sum_b = b->h_sums[2 * i];
+        out->sum_eb = b->h_sums[2 * i + 1];
+        out->n = b->tasks[i].n_ins;
+    }
+    if (grid) *grid = b->grids[i];
+    return ST_OK;
+}
+
+}  // namespace er

@@ -1,0 +1,194 @@
+// er_runtime.h -- host runtime behind the C ABI: device context with a
+// caching HBM pool and admission accounting, resident frameworks, plain
+// (one framework per launch) grid and Widom runs, and batched runs captured
+// in a hipGraph.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include <map>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "er_common.h"
+#include "er_kernels.h"
+
+namespace er {
+
+// Status codes (mirrored by include/emptyroom.h).
+enum Status : int {
+    ST_OK = 0,
+    ST_ARG = 1,
+    ST_IO = 2,
+    ST_PARSE = 3,
+    ST_MISSING = 4,
+    ST_HIP = 5,
+    ST_NOMEM = 6,
+    ST_STATE = 7,
+    ST_CAPACITY = 8,
+    ST_NODEVICE = 9,
+};
+
+void set_error(const std::string &msg);
+const char *last_error();
+
+// Reference FrameworkParticle layout (src/datatypes.h:20-28).
+struct Particle {
+    double x_f, y_f, z_f, eps, sig2;
+};
+
+// Kernel shape knobs.  The list capacity and the tile shape decide the order
+// of every FP64 sum, so a framework always runs with the same knobs whether it
+// runs alone or inside a batch.
+struct Tuning {
+    int ppl = 1;                 // points per lane (1 or 2)
+    int cap = 1536;              // LDS list capacity (records of 32 B)
+    int tile[3] = {8, 8, 4};     // grid tile in points, product = 256 * ppl
+    int widom_per_bin = 256;     // target insertions per Widom bin
+    double s2_switch = 0.25;     // (sigma/r)^2 above which a pair takes the FP64 path
+};
+
+class Context;
+
+class DevicePool {
+  public:
+    explicit DevicePool(Context *ctx) : ctx_(ctx) {}
+    ~DevicePool();
+    // Returns nullptr and sets the error if the cap would be exceeded.
+    void *alloc(size_t bytes);
+    void free(void *p);
+    size_t in_use() const { return in_use_; }
+    size_t reserved() const { return reserved_; }
+    void trim();
+
+  private:
+    static size_t round_size(size_t b);
+    Context *ctx_;
+    std::map<size_t, std::vector<void *>> cache_;
+    std::map<void *, size_t> live_;
+    size_t in_use_ = 0, reserved_ = 0;
+    std::mutex mu_;
+};
+
+class Context {
+  public:
+    int device = 0;
+    hipStream_t stream = nullptr;
+    size_t hbm_cap = 0;  // 0 = no cap beyond the device
+    int n_cu = 0;
+    Tuning tuning;
+    DevicePool pool{this};
+    unsigned long long *d_counter = nullptr;  // work counter of plain launches
+    double2 *d_partials = nullptr;
+    double *d_sums = nullptr;
+    ~Context();
+};
+
+int context_create(int device, size_t hbm_cap, Context **out);
+
+struct FrameworkDev {
+    Context *ctx = nullptr;
+    int n_atoms = 0, ntypes = 0;
+    double t[9], inv_t[9], rc2 = 0, rc = 0, volume = 0;
+    int rep[3] = {0, 0, 0};
+    long long n_images = 0;
+    size_t near_capacity = 0;
+    std::vector<TypeParams> types;
+    AtomDev *d_atoms = nullptr;
+    int *d_type_atom0 = nullptr, *d_type_block0 = nullptr, *d_type_begin = nullptr;
+    TypeParams *d_types = nullptr;
+    double4 *d_near = nullptr;
+    uint32_t *d_block_count = nullptr, *d_block_off = nullptr;
+    std::vector<void *> owned;
+    ~FrameworkDev();
+};
+
+int framework_upload(Context *ctx, const Particle *atoms, int n, const double t[9],
+                     const int rep[3], double rc2, FrameworkDev **out);
+
+struct GridDev {
+    Context *ctx = nullptr;
+    const FrameworkDev *fw = nullptr;
+    int n[3] = {0, 0, 0};
+    int precision = 0;  // 0 FP32, 1 FP64
+    void *d_data = nullptr;
+    size_t bytes() const;
+    ~GridDev();
+};
+
+struct WidomResult {
+    double sum_b = 0, sum_eb = 0;
+    uint64_t n = 0;
+};
+
+// Plain path: one framework per launch, synchronous.
+int grid_compute(Context *ctx, FrameworkDev *fw, const int n[3], int precision, void *host_out,
+                 GridDev **resident);
+int widom_run(Context *ctx, FrameworkDev *fw, uint64_t n_ins, double temperature, uint64_t seed,
+              WidomResult *out, double *energies_host);
+// In-cutoff (point, atom, image) triples of a grid / a Widom run.
+int count_incut_grid(Context *ctx, FrameworkDev *fw, const int n[3], unsigned long long *out);
+int count_incut_widom(Context *ctx, FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
+                      unsigned long long *out);
+
+// Device buffers and job descriptors of one Widom job.
+struct WidomBuffers {
+    int nbins[3] = {1, 1, 1};
+    uint32_t *count = nullptr, *start = nullptr, *cursor = nullptr, *sorted = nullptr;
+    double *e = nullptr;
+};
+void choose_widom_bins(const FrameworkDev *fw, uint64_t n_ins, const Tuning &tu, int nb[3]);
+void choose_grid_tile(const FrameworkDev *fw, const int n[3], const Tuning &tu, int tile[3]);
+JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void *d_out,
+                     const Tuning &tu, int math);
+JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
+                      const WidomBuffers &wb);
+
+// Batched path: many (framework, grid, Widom) tasks in one persistent launch,
+// captured in a hipGraph on first run and replayed.
+struct Task {
+    FrameworkDev *fw = nullptr;
+    int grid_n[3] = {0, 0, 0};  // all zero: no grid
+    int precision = 0;
+    uint64_t n_ins = 0;         // 0: no Widom
+    double temperature = 298.0;
+    uint64_t seed = 0;
+};
+
+class Batch {
+  public:
+    Context *ctx = nullptr;
+    std::vector<Task> tasks;
+    std::vector<GridDev *> grids;      // per task (nullptr if no grid)
+    std::vector<WidomBuffers> wbufs;   // per task
+    std::vector<JobDev> jobs;          // all jobs, cost-ordered
+    JobDev *d_jobs = nullptr;
+    long long total_items = 0;
+    int math = MATH_MIXED;
+    double2 *d_partials = nullptr;     // kReduceBlocks per task
+    double *d_sums = nullptr;          // 2 per task
+    unsigned long long *d_counter = nullptr;
+    double *h_sums = nullptr;          // pinned
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t exec = nullptr;
+    bool use_graph = true;
+    // eager runs only: events around each tiled-kernel launch of the last run
+    bool timing = false;
+    hipEvent_t ev[4] = {nullptr, nullptr, nullptr, nullptr};
+    int timed_launches = 0;
+    std::vector<void *> owned;
+    ~Batch();
+    int record(hipStream_t s);  // enqueue the whole batch on s
+};
+
+int batch_create(Context *ctx, const Task *tasks, int n, bool use_graph, Batch **out);
+int batch_run(Batch *b);   // asynchronous
+int batch_wait(Batch *b);  // waits and fetches the Widom sums
+int batch_result(Batch *b, int i, WidomResult *out, GridDev **grid);
+int batch_set_timing(Batch *b, bool on);
+// Milliseconds of the tiled-kernel launches of the last eager run.
+int batch_kernel_ms(Batch *b, double *ms, int *launches);
+
+}  // namespace er

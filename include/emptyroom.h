@@ -1,0 +1,214 @@
+/*
+ * emptyroom.h -- C ABI of libemptyroom.so, the MI355X (gfx950) implementation
+ * of EmptyRoom's two GPU hot paths: the guest-framework Lennard-Jones energy
+ * grid (writegrid) and Widom-insertion Henry coefficients (henry).
+ *
+ * Plain C, POD arguments only, no exceptions across the boundary.  Every call
+ * returns an er_status (0 = ER_OK); on failure er_last_error() holds a
+ * message for the calling thread.  One context serves one host thread at a
+ * time; the server API serialises internally.
+ *
+ * Reference interfaces replaced (paths relative to the EmptyRoom tree):
+ *   ComputeGridSheet + the x-sheet loop   src/computegridsheet.h:30-99,
+ *                                         src/writegrid.cu:175-271
+ *   doHenry + launch + block reduction    src/doHenry.h:29-135,
+ *                                         src/henry.cu:111-169
+ *   FrameworkParticle                     src/datatypes.h:20-28
+ *   host loaders and .out writers         src/Framework.cc, src/Forcefield.cc,
+ *                                         src/readsettings.h,
+ *                                         src/load_fast_particle_f_array.h,
+ *                                         src/write_settings_to_outputfile.h
+ */
+#ifndef EMPTYROOM_H_
+#define EMPTYROOM_H_
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef enum er_status {
+    ER_OK = 0,
+    ER_E_ARG = 1,       /* invalid argument */
+    ER_E_IO = 2,        /* file missing or unreadable */
+    ER_E_PARSE = 3,     /* malformed input file */
+    ER_E_MISSING = 4,   /* required key, label or mass not found */
+    ER_E_HIP = 5,       /* HIP runtime error */
+    ER_E_NOMEM = 6,     /* allocation failed or HBM cap reached */
+    ER_E_STATE = 7,     /* call not valid in the object's current state */
+    ER_E_CAPACITY = 8,  /* server queue full */
+    ER_E_NODEVICE = 9   /* no HIP device */
+} er_status;
+
+typedef enum er_precision {
+    ER_FP32 = 0, /* FP32 grid: FP32 LJ for pairs beyond 2 sigma, FP64 r^2, close pairs and sums */
+    ER_FP64 = 1  /* FP64 grid: every pair on the FP64 path (the reference's arithmetic) */
+} er_precision;
+
+typedef struct er_context er_context;
+typedef struct er_framework er_framework;
+typedef struct er_grid er_grid;
+typedef struct er_batch er_batch;
+typedef struct er_server er_server;
+typedef struct er_host_run er_host_run;
+
+/* One framework atom after mixing with the guest: fractional position and
+ * eps_ij (K), sigma_ij^2 (A^2).  Same layout as the reference's
+ * FrameworkParticle (src/datatypes.h:20-28). */
+typedef struct er_particle {
+    double x_f, y_f, z_f;
+    double eps_with_adsorbate;
+    double sig_with_adsorbate_squared;
+} er_particle;
+
+/* Widom sums: sum of exp(-E/T) and of E*exp(-E/T) over n insertions, the two
+ * fields of HenryStats the host reduces (src/datatypes.h:162-168,
+ * src/henry.cu:157-164). */
+typedef struct er_widom_result {
+    double sum_b;
+    double sum_eb;
+    uint64_t n;
+} er_widom_result;
+
+/* Kernel shape knobs (see DESIGN notes in the repository). */
+typedef struct er_tuning {
+    int points_per_lane;   /* 1 or 2 */
+    int lds_records;       /* LDS candidate-list capacity, 512..4800 records of 32 B */
+    int tile[3];           /* grid tile in points, product = 256 * points_per_lane */
+    int widom_per_bin;     /* target insertions per Widom bin */
+    double s2_switch;      /* (sigma/r)^2 above which an FP32-grid pair uses FP64 */
+} er_tuning;
+
+const char *er_last_error(void);
+const char *er_version(void);
+
+/* ---- context ------------------------------------------------------------ */
+/* hbm_cap_bytes: 0 = whole device; otherwise device allocations of this
+ * context (frameworks, grids, scratch) are refused beyond the cap. */
+int er_context_create(int device, size_t hbm_cap_bytes, er_context **out);
+int er_context_destroy(er_context *ctx);
+int er_context_get_tuning(const er_context *ctx, er_tuning *out);
+int er_context_set_tuning(er_context *ctx, const er_tuning *tuning);
+int er_context_memory(const er_context *ctx, size_t *in_use, size_t *reserved);
+
+/* ---- frameworks ----------------------------------------------------------
+ * Replaces the atom upload of src/writegrid.cu:186-189 / src/henry.cu:133-135
+ * and the by-value GridParameters/HenryParameters fields the kernels read
+ * (t_matrix, replication factors, r_cutoff_squared; src/datatypes.h:44-110).
+ * t_matrix is row-major fractional->Cartesian (src/Framework.cc:124-132);
+ * rep[3] are the image half-ranges of the "once" .uc file; rc2 = r_c^2.
+ * The device copy stays resident until er_framework_release. */
+int er_framework_upload(er_context *ctx, const er_particle *atoms, int n_atoms,
+                        const double t_matrix[9], const int rep[3], double rc2,
+                        er_framework **out);
+int er_framework_release(er_framework *fw);
+
+/* ---- energy grid (plain path) ---------------------------------------------
+ * Replaces the whole x-sheet loop of src/writegrid.cu:219-271 around
+ * ComputeGridSheet (src/computegridsheet.h:30-99): the energy (K) at every
+ * fractional point (i/(nx-1), j/(ny-1), k/(nz-1)), stored at
+ * k + j*nz + i*ny*nz (src/computegridsheet.h:94-97).  host_out (nullable)
+ * receives float[nx*ny*nz] for ER_FP32 or double[...] for ER_FP64;
+ * resident_out (nullable) keeps the device grid.  FP32 values at or above
+ * FLT_MAX are stored as FLT_MAX. */
+int er_grid_compute(er_context *ctx, er_framework *fw, const int n[3], int precision,
+                    void *host_out, er_grid **resident_out);
+int er_grid_download(er_grid *grid, void *host_out);
+int er_grid_info(const er_grid *grid, int n[3], int *precision);
+int er_grid_release(er_grid *grid);
+
+/* ---- Widom insertion (plain path) ------------------------------------------
+ * Replaces doHenry and its launch (src/doHenry.h:29-135,
+ * src/henry.cu:111-164): n_insertions points from the Philox4x32-10 stream
+ * keyed by `seed` (the reference draws from cuRAND MTGP32 seeded 1234),
+ * guest-framework energy with the r^2 clamp 1e-32, FP64 exp(-E/T) and FP64
+ * sums in a fixed order.  energies_out (nullable) receives E (K) per
+ * insertion index.  K_H and <E> follow from er_henry_result. */
+int er_widom(er_context *ctx, er_framework *fw, uint64_t n_insertions, double temperature,
+             uint64_t seed, er_widom_result *out, double *energies_out);
+
+/* Number of (point, atom, image) triples with r^2 < r_c^2: the work unit of
+ * the roofline figures.  Uses the same tile lists as the energy kernels. */
+int er_count_incut_grid(er_context *ctx, er_framework *fw, const int n[3], uint64_t *out);
+int er_count_incut_widom(er_context *ctx, er_framework *fw, uint64_t n_insertions, uint64_t seed,
+                         uint64_t *out);
+
+/* K_H (mol/(kg Pa) and mol/(m^3 Pa)) and <E> (K, kJ/mol) from Widom sums,
+ * src/henry.cu:166-174. */
+void er_henry_result(double sum_b, double sum_eb, double n_total, double temperature,
+                     double density, double *kh_mol_per_kg_pa, double *kh_mol_per_m3_pa,
+                     double *e_avg_k, double *e_avg_kj_per_mol);
+
+/* ---- batched path ----------------------------------------------------------
+ * No reference counterpart (the reference runs one framework per process).
+ * Many (framework, grid, Widom) tasks run as one persistent launch; with
+ * use_graph the whole batch is captured in a hipGraph on the first run and
+ * replayed afterwards.  Results are identical to the plain path. */
+typedef struct er_task {
+    er_framework *fw;
+    int grid_n[3];          /* all zero: no grid */
+    int precision;          /* er_precision of the grid */
+    uint64_t n_insertions;  /* 0: no Widom */
+    double temperature;
+    uint64_t seed;
+} er_task;
+
+int er_batch_create(er_context *ctx, const er_task *tasks, int n_tasks, int use_graph,
+                    er_batch **out);
+int er_batch_run(er_batch *batch);   /* asynchronous */
+int er_batch_wait(er_batch *batch);  /* blocks, then results are valid */
+/* grid_out (nullable) receives a view owned by the batch (do not release). */
+int er_batch_result(er_batch *batch, int task, er_widom_result *out, er_grid **grid_out);
+int er_batch_release(er_batch *batch);
+/* Eager runs (use_graph = 0) only: bracket every tiled-kernel launch with HIP
+ * events on the library's stream; er_batch_kernel_ms then returns the summed
+ * milliseconds of those launches in the last run and how many there were. */
+int er_batch_set_timing(er_batch *batch, int on);
+int er_batch_kernel_ms(er_batch *batch, double *ms, int *launches);
+
+/* ---- host side of the drop-in binaries -------------------------------------
+ * Everything src/writegrid.cu:64-127 / src/henry.cu:50-95 do before the
+ * device work, from a directory laid out like the reference (simulation.input,
+ * data/...).  On error the return code is set and er_last_error() holds the
+ * message the reference prints. mode: 0 writegrid, 1 henry. */
+typedef struct er_host_info {
+    double a, b, c, alpha, beta, gamma; /* lengths (A), angles (radians) */
+    int n_atoms;
+    double density;                     /* kg/m^3 */
+    double volume;                      /* A^3 */
+    double t_matrix[9], inv_t_matrix[9];
+    int rep[3];
+    double r_cutoff_squared;
+    double temperature;
+    double epsilon_guest, sigma_guest, adsorbate_mw;
+    int grid_n[3];                      /* writegrid */
+    double grid_resolution;             /* writegrid */
+    int grid_precision;                 /* writegrid: GridPrecision extension key */
+    int insertions_per_a3;              /* henry */
+    int numinsertions;                  /* henry: (int)(per_A3 * V) */
+    int insertions_per_thread;          /* henry: numinsertions/16384 + 1 */
+    int64_t total_insertions;           /* henry: 16384 * insertions_per_thread */
+    uint64_t seed;                      /* henry: WidomSeed extension key, default 1234 */
+} er_host_info;
+
+int er_host_setup(const char *root, const char *structure, const char *adsorbate, int mode,
+                  er_host_run **out);
+int er_host_run_info(const er_host_run *run, er_host_info *out);
+int er_host_run_particles(const er_host_run *run, er_particle *out, int capacity);
+/* label of atom i (NUL-terminated, truncated to capacity) */
+int er_host_run_label(const er_host_run *run, int i, char *out, int capacity);
+int er_host_run_write_settings(const er_host_run *run, const char *path);
+void er_host_run_release(er_host_run *run);
+
+/* Grid file writer of src/writegrid.cu:144-171,244-269 ("txt" or "cube");
+ * exactly one of e64 / e32 is non-null. */
+int er_write_grid_file(const char *path, const char *format, const double *e64, const float *e32,
+                       const int n[3], const double t_matrix[9], int threads);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* EMPTYROOM_H_ */
