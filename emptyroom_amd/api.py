@@ -79,7 +79,7 @@ EXPORTS = [
     "er_context_get_tuning", "er_context_set_tuning", "er_context_memory",
     "er_framework_upload", "er_framework_release", "er_grid_compute", "er_grid_download",
     "er_grid_info", "er_grid_release", "er_widom", "er_count_incut_grid",
-    "er_count_incut_widom", "er_henry_result", "er_batch_create", "er_batch_run",
+    "er_count_incut_widom", "er_widom_point", "er_henry_result", "er_batch_create", "er_batch_run",
     "er_batch_wait", "er_batch_result", "er_batch_release", "er_batch_set_timing",
     "er_batch_kernel_ms", "er_host_setup",
     "er_host_run_info", "er_host_run_particles", "er_host_run_label",
@@ -115,6 +115,8 @@ _SIGS = {
                                            ctypes.POINTER(ctypes.c_uint64)]),
     "er_count_incut_widom": (ctypes.c_int, [_P, _P, ctypes.c_uint64, ctypes.c_uint64,
                                             ctypes.POINTER(ctypes.c_uint64)]),
+    "er_widom_point": (None, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
+                              ctypes.POINTER(ctypes.c_double)]),
     "er_henry_result": (None, [ctypes.c_double] * 5 + [ctypes.POINTER(ctypes.c_double)] * 4),
     "er_batch_create": (ctypes.c_int, [_P, ctypes.POINTER(Task), ctypes.c_int, ctypes.c_int,
                                        ctypes.POINTER(_P)]),
@@ -266,6 +268,13 @@ def count_incut_widom(ctx: Context, fw: Framework, n: int, seed: int) -> int:
     v = ctypes.c_uint64()
     _check(lib().er_count_incut_widom(ctx.h, fw.h, int(n), int(seed), ctypes.byref(v)))
     return v.value
+
+
+def widom_point(seed: int, stream: int, i: int) -> np.ndarray:
+    """Fractional coordinates of insertion i of stream (seed, stream)."""
+    f = np.empty(3)
+    lib().er_widom_point(int(seed), int(stream), int(i), _dptr(f))
+    return f
 
 
 def henry_result(sum_b: float, sum_eb: float, n_total: float, temperature: float,

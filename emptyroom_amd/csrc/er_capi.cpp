@@ -202,6 +202,10 @@ int er_count_incut_widom(er_context *ctx, er_framework *fw, uint64_t n_insertion
     });
 }
 
+void er_widom_point(uint64_t seed, uint32_t stream, uint64_t i, double f[3]) {
+    if (f) er::widom_point(seed, stream, i, f);
+}
+
 void er_henry_result(double sum_b, double sum_eb, double n_total, double temperature,
                      double density, double *kh_mol_per_kg_pa, double *kh_mol_per_m3_pa,
                      double *e_avg_k, double *e_avg_kj_per_mol) {

@@ -129,6 +129,11 @@ int er_grid_release(er_grid *grid);
 int er_widom(er_context *ctx, er_framework *fw, uint64_t n_insertions, double temperature,
              uint64_t seed, er_widom_result *out, double *energies_out);
 
+/* Fractional coordinates in (0,1] of insertion i of the stream (seed, stream):
+ * stream 0 is er_widom's.  Lets a caller replay the exact points of a run
+ * (the reference's MTGP32 draws, src/doHenry.h:53-55, cannot be replayed). */
+void er_widom_point(uint64_t seed, uint32_t stream, uint64_t i, double f[3]);
+
 /* Number of (point, atom, image) triples with r^2 < r_c^2: the work unit of
  * the roofline figures.  Uses the same tile lists as the energy kernels. */
 int er_count_incut_grid(er_context *ctx, er_framework *fw, const int n[3], uint64_t *out);

@@ -1,0 +1,82 @@
+"""The drop-in writegrid / henry binaries on the GPU, end to end on the
+reference's IRMOF-1 inputs: grid file and .out report against the oracle."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from emptyroom_amd import api
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+GOLDEN = os.path.join(HERE, "golden", "irmof1")
+
+
+def workdir(tmp_path, settings):
+    d = tmp_path / "run"
+    shutil.copytree(GOLDEN, d)
+    (d / "output_files").mkdir()
+    (d / "data" / "grids").mkdir()
+    text = (d / "simulation.input").read_text()
+    for k, v in settings.items():
+        text = re.sub(rf"^{k} .*$", f"{k} {v}", text, flags=re.M)
+        if not re.search(rf"^{k} ", text, flags=re.M):
+            text += f"\n{k} {v}\n"
+    (d / "simulation.input").write_text(text)
+    return d
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_writegrid_irmof1(tmp_path, precision):
+    print(f"writegrid IRMOF-1 Kr {precision}", flush=True)
+    d = workdir(tmp_path, {"GridResolution": "1.0", "GridPrecision": precision})
+    p = subprocess.run([os.path.join(api.BIN_DIR, "writegrid"), "IRMOF-1_clean", "Kr"], cwd=d,
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    lines = (d / "data" / "grids" / "IRMOF-1_clean_Kr_UFF.txt").read_text().split("\n")
+    n = [int(x) for x in lines[0].split()[:3]]
+    assert n == [26, 26, 26]
+    assert lines[0].endswith("grid points (grid is in fractional coords). Endpoints included.")
+    rows = lines[1:-1]
+    assert len(rows) == n[0] * n[1] and lines[-1] == ""
+    vals = np.array([[float(x) for x in r.split()] for r in rows]).reshape(n)
+    hr = api.HostRun(str(d), "IRMOF-1_clean", "Kr", 0)
+    info = hr.info()
+    atoms = hr.particles()
+    ref = oracle.lj_grid(np.array(info.t_matrix).reshape(3, 3), atoms, list(info.rep),
+                         info.r_cutoff_squared, n)
+    # text holds 7 significant digits: agreement to half a unit in the last one
+    rel = np.abs(vals - ref) / np.maximum(np.abs(ref), 1e-30)
+    tol = 5.0000001e-7 if precision == "fp64" else 1e-6
+    m = ref < 1e4 if precision == "fp32" else np.ones_like(ref, bool)
+    assert np.all((rel[m] <= tol) | (np.abs(vals - ref)[m] <= 1e-3))
+    out = (d / "output_files" / "IRMOF-1_clean_Kr_grid.out").read_text()
+    assert "Grid: 26 by 26 by 26 points. Total grid points = 17576" in out
+    assert "Time to write grid:" in out
+
+
+def test_henry_irmof1(tmp_path):
+    print("henry IRMOF-1 Kr", flush=True)
+    d = workdir(tmp_path, {"NumberOfInsertionsPerA3": "2"})
+    p = subprocess.run([os.path.join(api.BIN_DIR, "henry"), "IRMOF-1_clean", "Kr"], cwd=d,
+                       capture_output=True, text=True, timeout=120)
+    assert p.returncode == 0, p.stdout + p.stderr
+    out = (d / "output_files" / "IRMOF-1_clean_Kr_henry.out").read_text()
+    total = int(re.search(r"Total # of insertions: (\d+)", out).group(1))
+    assert total == 16384 * (int(2 * 17237.492730368002) // 16384 + 1)
+    kh = float(re.search(r"Henry coefficient \(mol/\(kg-Pa\)\): (\S+)", out).group(1))
+    e_k = float(re.search(r"<energy> \(K\): (\S+)", out).group(1))
+    hr = api.HostRun(str(d), "IRMOF-1_clean", "Kr", 1)
+    info = hr.info()
+    atoms = hr.particles()
+    sb, seb = oracle.widom(np.array(info.t_matrix).reshape(3, 3), atoms, list(info.rep),
+                           info.r_cutoff_squared, info.temperature, 1234, 0, total)
+    kh_ref, _, e_ref, _ = oracle.henry_result(sb, seb, float(total), info.temperature,
+                                              info.density)
+    assert abs(kh - kh_ref) / kh_ref < 1e-4
+    assert abs(e_k - e_ref) < 1e-3 * max(1.0, abs(e_ref))

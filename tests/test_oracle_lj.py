@@ -1,0 +1,213 @@
+"""Pins of the oracle's LJ grid / Widom arithmetic (CPU).
+
+The reference kernels (src/computegridsheet.h, src/doHenry.h) need CUDA and
+cannot be built here, so each property below is fixed by mathematics or by an
+independent formulation, chosen so that a plausible slip fails one of them:
+a dropped or doubled image (loop bounds), <= instead of < at the cutoff, a
+missing r^2 clamp, a transposed cell matrix, mixing by product instead of
+mean, a wrong sign in the LJ form.
+"""
+import math
+
+import numpy as np
+import pytest
+
+from emptyroom_amd import synth
+from oracle import oracle
+
+GRID_MIN_R2 = 1e-12 * 1e-12
+HENRY_MIN_R2 = 1e-16 * 1e-16
+
+
+def lj(r, eps, sigma):
+    s6 = (sigma / r) ** 6
+    return 4 * eps * (s6 * s6 - s6)
+
+
+def cubic(L):
+    return np.diag([L, L, L]).astype(float)
+
+
+def test_single_atom_closed_forms():
+    """One atom in a 40 A cell: only the home image is within 12.5 A."""
+    L, eps, sigma = 40.0, 120.0, 3.4
+    atoms = np.array([[0.5, 0.5, 0.5, eps, sigma * sigma]])
+    t = cubic(L)
+    for r, expect in [(sigma, 0.0), (2 ** (1 / 6) * sigma, -eps), (5.0, lj(5.0, eps, sigma)),
+                      (11.0, lj(11.0, eps, sigma))]:
+        f = np.array([0.5 + r / L, 0.5, 0.5])
+        e = oracle.lj_energy(t, atoms, [1, 1, 1], 12.5 ** 2, GRID_MIN_R2, f)
+        assert abs(e - expect) <= 1e-9 * eps, (r, e, expect)
+
+
+def test_cutoff_is_strict_and_images_counted():
+    """Atom at fractional 0.5 of a 25 A cubic cell: the point (0,0,0) sees the
+    home image and the -1 image at exactly r_c = 12.5 A."""
+    t = cubic(25.0)
+    eps, sig2 = 100.0, 9.0
+    atoms = np.array([[0.5, 0.0, 0.0, eps, sig2]])
+    f0 = np.zeros(3)
+    e_at = oracle.lj_energy(t, atoms, [1, 1, 1], 156.25, GRID_MIN_R2, f0)
+    assert e_at == 0.0  # r^2 == rc^2 is excluded (src/computegridsheet.h:79)
+    e_in = oracle.lj_energy(t, atoms, [1, 1, 1], 156.25 + 1e-9, GRID_MIN_R2, f0)
+    assert e_in == pytest.approx(2 * lj(12.5, eps, 3.0), rel=1e-12)
+    # with rep = 0 only the home image exists
+    e_home = oracle.lj_energy(t, atoms, [0, 0, 0], 156.25 + 1e-9, GRID_MIN_R2, f0)
+    assert e_home == pytest.approx(lj(12.5, eps, 3.0), rel=1e-12)
+
+
+def test_clamp_values():
+    """r^2 is clamped (not skipped) at 1e-24 on the grid and 1e-32 for Henry."""
+    t = cubic(30.0)
+    eps, sig2 = 50.0, 10.0
+    atoms = np.array([[0.25, 0.25, 0.25, eps, sig2]])
+    f = np.array([0.25, 0.25, 0.25])
+    for mr2 in (GRID_MIN_R2, HENRY_MIN_R2):
+        e = oracle.lj_energy(t, atoms, [0, 0, 0], 156.25, mr2, f)
+        s6 = (sig2 / mr2) ** 3
+        assert math.isfinite(e)
+        assert e == pytest.approx(4 * eps * s6 * (s6 - 1), rel=1e-12)
+
+
+def test_minimum_image_equivalence():
+    """Cubic 25 A cell, r_c = 12.5 <= L/2, one image shell: the reference's
+    enumeration equals the minimum-image sum written independently."""
+    f = synth.config1()
+    atoms = f.particles()
+    n = [7, 6, 5]
+    grid = oracle.lj_grid(f.t, atoms, [1, 1, 1], f.rc2, n)
+    L = 25.0
+    for i in range(n[0]):
+        for j in range(n[1]):
+            for k in range(n[2]):
+                p = np.array([i / (n[0] - 1), j / (n[1] - 1), k / (n[2] - 1)])
+                d = atoms[:, :3] - p
+                d -= np.round(d)
+                r2 = np.maximum((L * L) * np.sum(d * d, axis=1), GRID_MIN_R2)
+                m = r2 < f.rc2
+                s6 = (atoms[m, 4] / r2[m]) ** 3
+                ref = np.sum(4 * atoms[m, 3] * s6 * (s6 - 1))
+                assert grid[i, j, k] == pytest.approx(ref, rel=1e-10, abs=1e-9)
+
+
+def test_periodic_faces_equal():
+    """f = 0 and f = 1 are the same point once every in-cutoff image is enumerated."""
+    f = synth.small_framework(5, n_atoms=25, cell=(10.0, 11.0, 12.0, 80.0, 95.0, 105.0))
+    atoms = f.particles()
+    n = [6, 7, 8]
+    g = oracle.lj_grid(f.t, atoms, f.rep, f.rc2, n)
+    scale = np.maximum(1.0, np.abs(g))
+    assert np.all(np.abs(g[0] - g[-1]) / scale[0] < 1e-9)
+    assert np.all(np.abs(g[:, 0] - g[:, -1]) / scale[:, 0] < 1e-9)
+    assert np.all(np.abs(g[:, :, 0] - g[:, :, -1]) / scale[:, :, 0] < 1e-9)
+
+
+def test_permutation_and_translation_invariance():
+    f = synth.small_framework(6, n_atoms=20, cell=(12.0, 12.5, 13.0, 85.0, 92.0, 98.0))
+    atoms = f.particles()
+    rep = [r + 1 for r in f.rep]
+    pts = synth.Rng(3).uniform(30).reshape(10, 3) * 0.8 + 0.1
+    e0 = oracle.lj_points(f.t, atoms, rep, f.rc2, GRID_MIN_R2, pts)
+    perm = synth.Rng(4).u64(len(atoms)).argsort()
+    e1 = oracle.lj_points(f.t, atoms[perm], rep, f.rc2, GRID_MIN_R2, pts)
+    assert np.allclose(e0, e1, rtol=1e-11, atol=1e-9)
+    shift = np.array([0.03, -0.02, 0.05])
+    moved = atoms.copy()
+    moved[:, :3] += shift
+    e2 = oracle.lj_points(f.t, moved, rep, f.rc2, GRID_MIN_R2, pts + shift)
+    assert np.allclose(e0, e2, rtol=1e-9, atol=1e-7)
+
+
+def test_transposed_cell_matrix_is_detected():
+    """A skewed cell: the oracle must use T f (rows of t_matrix), not T^T f."""
+    t, inv, v = oracle.cell_matrices(10.0, 12.0, 14.0, 70.0, 100.0, 115.0)
+    a = np.array([10.0, 0, 0])
+    g = math.radians(115.0)
+    b = np.array([12 * math.cos(g), 12 * math.sin(g), 0.0])
+    assert np.allclose(t[:, 0], a) and np.allclose(t[:, 1], b)
+    assert v == pytest.approx(abs(np.linalg.det(t)), rel=1e-12)
+    assert np.allclose(inv @ t, np.eye(3), atol=1e-12)
+    eps, sig2 = 80.0, 10.0
+    atoms = np.array([[0.1, 0.2, 0.3, eps, sig2]])
+    p = np.array([0.5, 0.1, 0.1])
+    e = oracle.lj_energy(t, atoms, [0, 0, 0], 1e6, GRID_MIN_R2, p)
+    r = np.linalg.norm(t @ (p - atoms[0, :3]))
+    assert e == pytest.approx(lj(r, eps, math.sqrt(sig2)), rel=1e-12)
+    r_wrong = np.linalg.norm(t.T @ (p - atoms[0, :3]))
+    assert abs(r - r_wrong) > 0.1
+
+
+def test_lorentz_berthelot():
+    e, s2 = oracle.lb_mix(148.0, 3.73, 52.8, 3.43)
+    assert e == pytest.approx(math.sqrt(148.0 * 52.8), rel=1e-15)
+    assert s2 == pytest.approx(((3.73 + 3.43) / 2) ** 2, rel=1e-15)
+
+
+def test_image_enumeration_order_and_count():
+    t = cubic(10.0)
+    atoms = np.array([[0.1, 0.2, 0.3, 1.0, 1.0], [0.6, 0.7, 0.8, 1.0, 1.0]])
+    img = oracle.image_atoms(t, atoms, [1, 2, 0])
+    assert img.shape == (3 * 5 * 1 * 2, 3)
+    # first image is (-1, -2, 0) applied to atom 0, then atom 1
+    assert np.allclose(img[0], [(0.1 - 1) * 10, (0.2 - 2) * 10, 3.0])
+    assert np.allclose(img[1], [(0.6 - 1) * 10, (0.7 - 2) * 10, 8.0])
+    assert np.allclose(img[-1], [(0.6 + 1) * 10, (0.7 + 2) * 10, 8.0])
+
+
+def test_count_incut_matches_bruteforce():
+    f = synth.small_framework(8, n_atoms=15, cell=(9.0, 10.0, 11.0, 75.0, 100.0, 95.0))
+    atoms = f.particles()
+    n = [4, 5, 3]
+    img = oracle.image_atoms(f.t, atoms, f.rep)
+    cnt = 0
+    for i in range(n[0]):
+        for j in range(n[1]):
+            for k in range(n[2]):
+                p = f.t @ np.array([i / (n[0] - 1), j / (n[1] - 1), k / (n[2] - 1)])
+                cnt += int(np.sum(np.sum((img - p) ** 2, axis=1) < f.rc2))
+    assert oracle.count_incut_grid(f.t, atoms, f.rep, f.rc2, n) == cnt
+
+
+def test_widom_zero_epsilon_exact():
+    """eps = 0: E = 0 everywhere, sum_b = n exactly, K_H = 1/(8.314 T)/rho."""
+    f = synth.small_framework(9, n_atoms=10)
+    atoms = f.particles()
+    atoms[:, 3] = 0.0
+    sb, seb = oracle.widom(f.t, atoms, f.rep, f.rc2, 300.0, 5, 0, 777)
+    assert sb == 777.0 and seb == 0.0
+    kh_kg, kh_m3, e_k, _ = oracle.henry_result(sb, seb, 777.0, 300.0, 1234.5)
+    assert kh_m3 == pytest.approx(1.0 / (8.314 * 300.0), rel=1e-15)
+    assert kh_kg == pytest.approx(1.0 / (8.314 * 300.0) / 1234.5, rel=1e-15)
+    assert e_k == 0.0
+
+
+def test_widom_energies_are_energies_at_the_stream_points():
+    f = synth.small_framework(10, n_atoms=12)
+    atoms = f.particles()
+    sb, seb, e = oracle.widom(f.t, atoms, f.rep, f.rc2, 298.0, 42, 100, 140, energies=True)
+    pts = np.array([oracle.widom_point(42, 0, i) for i in range(100, 140)])
+    ref = oracle.lj_points(f.t, atoms, f.rep, f.rc2, HENRY_MIN_R2, pts)
+    assert np.array_equal(e, ref)
+    b = np.exp(-ref / 298.0)
+    assert sb == pytest.approx(b.sum(), rel=1e-14)
+    assert seb == pytest.approx((ref * b).sum(), rel=1e-12, abs=1e-300)
+
+
+def test_henry_counts_reference_answers():
+    # IRMOF-1, 1000 insertions per A^3, values printed by the reference's host code
+    assert oracle.henry_counts(1000, 17237.492730368002) == (17237492, 1053, 17252352)
+    # survey example: 1e5 requested -> 16384 * 7 = 114688 executed
+    ni, pt, tot = oracle.henry_counts(1, 100000.5)
+    assert (ni, pt, tot) == (100000, 7, 114688)
+
+
+def test_grid_boltzmann_average_close_to_widom():
+    """Loose physics check: the Boltzmann average over the non-duplicated grid
+    points and over uniform insertions estimate the same integral."""
+    f = synth.small_framework(12, n_atoms=6, cell=(14.0, 14.0, 14.0, 90.0, 90.0, 90.0))
+    atoms = f.particles()
+    n = [29, 29, 29]
+    g = oracle.lj_grid(f.t, atoms, f.rep, f.rc2, n)[:-1, :-1, :-1]
+    kg = np.mean(np.exp(-g / 298.0))
+    sb, _ = oracle.widom(f.t, atoms, f.rep, f.rc2, 298.0, 3, 0, 60000)
+    assert sb / 60000 == pytest.approx(kg, rel=0.05)

@@ -1,0 +1,174 @@
+"""Pins of the oracle's Philox stream, pocket blocking and trilinear
+interpolation (CPU).
+
+- Philox4x32-10: the Random123 known-answer vectors (Salmon et al., SC'11).
+- Pocket blocking: hand-built bitmaps whose answers follow from the
+  definition, and, where the reference tree is present, the reference's own
+  FindAndBlockPockets (src/pocketblocking.cc:126-391) built by oracle/Makefile
+  into oracle/_ref/ref_host, compared bit for bit on random grids.
+- Trilinear interpolation (src/gcmc.cc:145-177): exact on affine functions,
+  node values at nodes, wrap semantics of src/gcmc.cc:74-77.
+"""
+import os
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+from emptyroom_amd import api, synth
+from oracle import oracle
+
+KAT = [  # counter, key, expected output (Random123 kat_vectors, philox4x32 10 rounds)
+    ([0, 0, 0, 0], [0, 0], [0x6627E8D5, 0xE169C58D, 0xBC57AC4C, 0x9B00DBD8]),
+    ([0xFFFFFFFF] * 4, [0xFFFFFFFF] * 2, [0x408F276D, 0x41C83B0E, 0xA20BC7C6, 0x6D5451FD]),
+    ([0x243F6A88, 0x85A308D3, 0x13198A2E, 0x03707344], [0xA4093822, 0x299F31D0],
+     [0xD16CFE09, 0x94FDCCEB, 0x5001E420, 0x24126EA1]),
+]
+
+
+@pytest.mark.parametrize("ctr,key,out", KAT)
+def test_philox_known_answers(ctr, key, out):
+    assert oracle.philox(ctr, key) == out
+
+
+def test_widom_points_open_closed_unit_cube():
+    pts = np.array([oracle.widom_point(77, 0, i) for i in range(4000)])
+    assert pts.min() > 0.0 and pts.max() <= 1.0
+    assert abs(pts.mean() - 0.5) < 0.01
+    # counter (lo(i), hi(i), stream, 0), key (lo(seed), hi(seed)), (w + 1) / 2^32
+    seed, i = 0x123456789ABCDEF0, (5 << 32) + 17
+    w = oracle.philox([17, 5, 1, 0], [0x9ABCDEF0, 0x12345678])
+    f = oracle.widom_point(seed, 1, i)
+    assert list(f) == [(w[k] + 1.0) / 4294967296.0 for k in range(3)]
+
+
+def test_product_stream_equals_oracle_stream():
+    """The library's own Philox (er_common.h, host side) replays the oracle's points."""
+    api.lib()
+    for seed, stream, i in [(0, 0, 0), (1234, 0, 99), (2 ** 63 + 5, 1, 2 ** 33 + 7)]:
+        assert np.array_equal(api.widom_point(seed, stream, i),
+                              oracle.widom_point(seed, stream, i))
+
+
+# --------------------------------------------------------------------------
+# pocket blocking
+# --------------------------------------------------------------------------
+
+T = 298.0
+WALL = 1e5  # inaccessible (>= 15 T)
+
+
+def blank(n):
+    return np.full(n, WALL)
+
+
+def test_isolated_cavity_is_one_pocket():
+    g = blank((8, 8, 8))
+    g[3:5, 3:5, 3:5] = 0.0
+    n, blocked, seg = oracle.find_and_block_pockets(g, T)
+    assert n == 1
+    assert np.all(blocked[3:5, 3:5, 3:5] == 1e13)
+    assert np.all(seg[g >= 15 * T] == -1) and np.all(seg[3:5, 3:5, 3:5] == 0)
+
+
+def test_straight_channel_is_no_pocket():
+    g = blank((8, 7, 6))
+    g[:, 3, 2] = 0.0  # spans x from face to face
+    n, blocked, _ = oracle.find_and_block_pockets(g, T)
+    assert n == 0 and np.array_equal(blocked, g)
+
+
+def test_channel_plus_side_cavity():
+    g = blank((9, 9, 9))
+    g[:, 4, 4] = 0.0
+    g[2, 1, 1] = 0.0
+    n, blocked, seg = oracle.find_and_block_pockets(g, T)
+    assert n == 1
+    assert blocked[2, 1, 1] == 1e13 and blocked[5, 4, 4] == 0.0
+    # segment ids follow the raster order of each segment's first voxel
+    assert seg[0, 4, 4] == 0 and seg[2, 1, 1] == 1
+
+
+STAIR = [(5, 0), (5, 1), (4, 1), (4, 2), (3, 2), (3, 3), (2, 3), (2, 4), (1, 4), (1, 5)]
+
+
+def test_loop_through_two_faces():
+    """Segment A = {(0,0)} and a staircase segment B from (5,0) to (0,5) in one
+    layer of a 6x6x3 grid.  A links to B across the x face and across the y
+    face, a loop with net translation -X+Y: both are channel, no pocket.
+    Without the y-face link the graph is a tree: both are pockets."""
+    g = blank((6, 6, 3))
+    g[0, 0, 1] = 0.0
+    for x, y in STAIR + [(0, 5)]:
+        g[x, y, 1] = 0.0
+    n, _, seg = oracle.find_and_block_pockets(g, T)
+    assert seg[0, 0, 1] != seg[5, 0, 1]
+    assert n == 0
+    g[0, 5, 1] = WALL
+    n, blocked, _ = oracle.find_and_block_pockets(g, T)
+    assert n == 2 and blocked[0, 0, 1] == 1e13 and blocked[5, 0, 1] == 1e13
+
+
+def test_threshold_is_inclusive():
+    """E >= 15 T is inaccessible (src/pocketblocking.cc:138)."""
+    g = blank((5, 5, 5))
+    g[2, 2, 2] = 15 * T          # exactly at the threshold: blocked
+    n, _, seg = oracle.find_and_block_pockets(g, T)
+    assert n == 0 and seg[2, 2, 2] == -1
+    g[2, 2, 2] = np.nextafter(15 * T, 0)
+    n, blocked, _ = oracle.find_and_block_pockets(g, T)
+    assert n == 1 and blocked[2, 2, 2] == 1e13
+
+
+def random_grid(seed, n, fill):
+    r = synth.Rng(seed).uniform(int(np.prod(n))).reshape(n)
+    return np.where(r < fill, 0.0, WALL) + r  # distinct values below/above 15T
+
+
+@pytest.mark.skipif(not oracle.have_ref(), reason="oracle/_ref not built (no reference tree)")
+@pytest.mark.parametrize("seed,n,fill", [(1, (9, 8, 7), 0.45), (2, (12, 12, 12), 0.35),
+                                         (3, (10, 6, 11), 0.55), (4, (16, 15, 14), 0.3),
+                                         (5, (7, 7, 7), 0.6)])
+def test_pockets_bit_exact_vs_reference(seed, n, fill):
+    g = random_grid(seed, n, fill)
+    n_or, blocked, _ = oracle.find_and_block_pockets(g, T)
+    with tempfile.TemporaryDirectory() as d:
+        src, dst = os.path.join(d, "in.f64"), os.path.join(d, "out.f64")
+        g.astype(np.float64).tofile(src)
+        out = subprocess.run([oracle.REF_HOST, "pockets", *map(str, n), repr(T), src, dst],
+                             capture_output=True, text=True, check=True).stdout
+        n_ref = int(out.split()[1])
+        ref = np.fromfile(dst, dtype=np.float64).reshape(n)
+    assert n_or == n_ref
+    assert np.array_equal(blocked, ref)
+
+
+# --------------------------------------------------------------------------
+# trilinear interpolation
+# --------------------------------------------------------------------------
+
+def test_trilinear_exact_on_affine_functions():
+    n = (7, 9, 5)
+    f = [np.linspace(0, 1, k) for k in n]
+    X, Y, Z = np.meshgrid(*f, indexing="ij")
+    g = 3.0 + 2.0 * X - 5.0 * Y + 0.5 * Z + 1.5 * X * Y  # bilinear term is also exact
+    rng = synth.Rng(11)
+    for p in rng.uniform(60).reshape(20, 3):
+        v = oracle.trilinear(g, *p)
+        assert v == pytest.approx(3 + 2 * p[0] - 5 * p[1] + 0.5 * p[2] + 1.5 * p[0] * p[1],
+                                  abs=1e-12)
+
+
+def test_trilinear_nodes_and_upper_face():
+    g = synth.Rng(12).uniform(4 * 5 * 6).reshape(4, 5, 6)
+    assert oracle.trilinear(g, 1 / 3, 0.5, 0.2) == pytest.approx(g[1, 2, 1], abs=1e-14)
+    # f = 1 reads node N-1 (lower index clamped to N-2)
+    assert oracle.trilinear(g, 1.0, 1.0, 1.0) == pytest.approx(g[3, 4, 5], abs=1e-14)
+
+
+def test_wrap():
+    assert oracle.wrap(1.25) == 0.25
+    assert oracle.wrap(-0.25) == 0.75
+    assert oracle.wrap(1.0) == 0.0
+    assert oracle.wrap(-1e-20) == 1.0  # rounds up: hence the clamp above
