@@ -214,7 +214,10 @@ __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, long lon
 // rounding of the tile radius and distances can never misclassify a pair.
 constexpr double kMargin = 1e-6;
 
-enum : int { CAT_NONE = 0, CAT_FAR = 1, CAT_FULL = 2 };
+enum : int { CAT_NONE = 0, CAT_FAR = 1, CAT_SHELL = 2, CAT_NEAR = 3 };
+
+// Ints per list segment: type, far begin/end, shell begin/end, near begin/end.
+constexpr int kSegInts = 7;
 
 struct Tile {
     double cx, cy, cz, R;
@@ -238,12 +241,25 @@ __device__ __forceinline__ Tile tile_sphere(const double *t, const double lo[3],
     return s;
 }
 
+// Split of the LDS list: [0, capA) holds far records (from 0 upwards) and
+// shell records (from capA-1 downwards); [capA, cap) holds near records.
+template <int MATH>
+__device__ __forceinline__ int near_region_start(int cap) {
+    if (MATH == MATH_FP64) return kWG;        // every record is near
+    if (MATH == MATH_COUNT) return cap - kWG; // no near records
+    // both regions hold at least one chunk of kWG records (cap >= 2 kWG), so a
+    // fill pass that starts empty always makes progress
+    const int near_cap = (cap / 3) & ~63;
+    return cap - (near_cap < kWG ? kWG : near_cap);
+}
+
 // Stream the near list of job J from (cur_t, cur_pos) into the LDS list until
-// it is exhausted or the list is full.  Far records grow from index 0, full
-// records from index cap-1 downwards.  Records keep the near-list order, so
-// the list (and every sum over it) depends only on the tile.  Returns the
-// number of segments written to seg (5 ints each: type, far begin/end, full
-// begin/end).
+// it is exhausted or a region is full, classifying each image atom by its
+// distance d to the tile centre: far (every point of the tile is inside the
+// cutoff and beyond the FP64 switch radius), shell (beyond the switch radius
+// but possibly outside the cutoff), near (possibly inside the switch radius).
+// Records keep the near-list order, so the list (and every sum over it)
+// depends only on the tile.  Returns the number of segments written to seg.
 template <int MATH>
 __device__ int fill_list(const JobDev *__restrict__ J, const Tile &tl, int &cur_t, int &cur_pos,
                          int cap, double4 *__restrict__ L, int *__restrict__ seg,
@@ -251,81 +267,113 @@ __device__ int fill_list(const JobDev *__restrict__ J, const Tile &tl, int &cur_
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int ntypes = J->ntypes;
     const double rc = J->rc;
-    int nfar = 0, nfull = 0, nseg = 0, far0 = 0, full0 = 0;
+    const int capA = near_region_start<MATH>(cap);
+    const int capN = cap - capA;
+    // Squared-distance forms of the tests on d = |image atom - tile centre|:
+    //   some point may be inside r_c:     d - R <  r_c             (in2)
+    //   every point is inside r_c:        d + R <  r_c             (far2)
+    //   some point may be inside r_FP64:  d - R <= r_FP64          (near2, per type)
+    // each widened by kMargin so FP64 rounding cannot misclassify.
+    const double in2 = (rc + tl.R + kMargin) * (rc + tl.R + kMargin);
+    const double far_r = rc - tl.R - kMargin;
+    const double far2 = far_r > 0.0 ? far_r * far_r : -1.0;
+    double near2 = -1.0;
+    int near_t = -1;
+    int nf = 0, ns = 0, nn = 0, nseg = 0, f0 = 0, s0 = 0, n0 = 0;
     while (cur_t < ntypes) {
+        if (MATH == MATH_MIXED && near_t != cur_t) {
+            const double rn = J->types[cur_t].rclose + tl.R + kMargin;
+            near2 = rn * rn;
+            near_t = cur_t;
+        }
         const int tend = uniform(J->type_begin[cur_t + 1]);
         if (cur_pos >= tend) {
-            if (nfar > far0 || nfull > full0) {
+            if (nf > f0 || ns > s0 || nn > n0) {
                 if (tid == 0) {
-                    int *s = seg + 5 * nseg;
-                    s[0] = cur_t; s[1] = far0; s[2] = nfar; s[3] = full0; s[4] = nfull;
+                    int *g = seg + kSegInts * nseg;
+                    g[0] = cur_t; g[1] = f0; g[2] = nf; g[3] = s0; g[4] = ns; g[5] = n0; g[6] = nn;
                 }
                 ++nseg;
             }
-            far0 = nfar;
-            full0 = nfull;
+            f0 = nf;
+            s0 = ns;
+            n0 = nn;
             ++cur_t;
             if (cur_t < ntypes) cur_pos = uniform(J->type_begin[cur_t]);
             continue;
         }
-        if (nfar + nfull + kWG > cap || nseg >= kMaxSeg) break;
+        if (nf + ns + kWG > capA || nn + kWG > capN || nseg >= kMaxSeg) break;
         const int idx = cur_pos + tid;
         int cat = CAT_NONE;
         double4 A = make_double4(0.0, 0.0, 0.0, 0.0);
         if (idx < tend) {
             A = J->near[idx];
             const double dx = A.x - tl.cx, dy = A.y - tl.cy, dz = A.z - tl.cz;
-            const double d = sqrt(dx * dx + dy * dy + dz * dz);
-            if (d - tl.R < rc + kMargin) {
-                cat = CAT_FULL;
-                if (MATH == MATH_MIXED) {
-                    const double rclose = J->types[cur_t].rclose;
-                    if (d + tl.R < rc - kMargin && d - tl.R > rclose + kMargin) cat = CAT_FAR;
-                } else if (MATH == MATH_COUNT) {
-                    if (d + tl.R < rc - kMargin) cat = CAT_FAR;
-                }
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            if (d2 < in2) {
+                if (MATH == MATH_FP64) cat = CAT_NEAR;
+                else if (MATH == MATH_MIXED && d2 <= near2) cat = CAT_NEAR;
+                else cat = d2 < far2 ? CAT_FAR : CAT_SHELL;
             }
         }
         const uint64_t bf = __ballot(cat == CAT_FAR);
-        const uint64_t bu = __ballot(cat == CAT_FULL);
+        const uint64_t bs = __ballot(cat == CAT_SHELL);
+        const uint64_t bn = __ballot(cat == CAT_NEAR);
         if (lane == 0) {
-            s_cnt[2 * wave] = __popcll(bf);
-            s_cnt[2 * wave + 1] = __popcll(bu);
+            s_cnt[3 * wave] = __popcll(bf);
+            s_cnt[3 * wave + 1] = __popcll(bs);
+            s_cnt[3 * wave + 2] = __popcll(bn);
         }
         __syncthreads();
-        int off_f = nfar, off_u = nfull, tot_f = 0, tot_u = 0;
+        int of = nf, os = ns, on = nn, tf = 0, ts = 0, tn = 0;
 #pragma unroll
         for (int w = 0; w < kWG / 64; ++w) {
-            const int cf = s_cnt[2 * w], cu = s_cnt[2 * w + 1];
+            const int cf = s_cnt[3 * w], cs = s_cnt[3 * w + 1], cn = s_cnt[3 * w + 2];
             if (w < wave) {
-                off_f += cf;
-                off_u += cu;
+                of += cf;
+                os += cs;
+                on += cn;
             }
-            tot_f += cf;
-            tot_u += cu;
+            tf += cf;
+            ts += cs;
+            tn += cn;
         }
-        if (cat == CAT_FAR) L[off_f + lanes_below(bf)] = A;
-        if (cat == CAT_FULL) L[cap - 1 - (off_u + lanes_below(bu))] = A;
-        nfar = uniform(nfar + tot_f);
-        nfull = uniform(nfull + tot_u);
+        if (cat == CAT_FAR) L[of + lanes_below(bf)] = A;
+        if (cat == CAT_SHELL) L[capA - 1 - (os + lanes_below(bs))] = A;
+        if (cat == CAT_NEAR) L[capA + on + lanes_below(bn)] = A;
+        nf = uniform(nf + tf);
+        ns = uniform(ns + ts);
+        nn = uniform(nn + tn);
         cur_pos += kWG;
         __syncthreads();
     }
-    if (nfar > far0 || nfull > full0) {
+    if (nf > f0 || ns > s0 || nn > n0) {
         if (tid == 0) {
-            int *s = seg + 5 * nseg;
-            s[0] = cur_t; s[1] = far0; s[2] = nfar; s[3] = full0; s[4] = nfull;
+            int *g = seg + kSegInts * nseg;
+            g[0] = cur_t; g[1] = f0; g[2] = nf; g[3] = s0; g[4] = ns; g[5] = n0; g[6] = nn;
         }
         ++nseg;
     }
     return nseg;
 }
 
+// FP32 LJ of one far/shell pair: FP64 displacement and r^2, FP32 reciprocal,
+// sigma^2/r^2, s^6 and s^6 (s^6 - 1); returns the pair energy / (4 eps).
+__device__ __forceinline__ float lj32(double px, double py, double pz, const double4 &A,
+                                      float sig2f, double &r2) {
+    const double dx = px - A.x, dy = py - A.y, dz = pz - A.z;
+    r2 = dx * dx + dy * dy + dz * dz;
+    const float inv = __builtin_amdgcn_rcpf((float)r2);
+    const float s2 = sig2f * inv;
+    const float s6 = s2 * s2 * s2;
+    return __builtin_fmaf(s6, s6, -s6);
+}
+
 // Energies (or in-cutoff counts) of this lane's PPL points against the LDS
-// list.  Far records: FP64 displacement and r^2, FP32 reciprocal and LJ,
-// FP32 partials flushed into the FP64 sum every 4 records.  Full records:
-// FP64 r^2, the reference's clamp and strict cutoff test, FP64 LJ with a
-// Newton-refined reciprocal.
+// list.  Far pairs: FP32 LJ, no cutoff test.  Shell pairs: FP32 LJ behind the
+// FP64 strict cutoff test.  FP32 partials are flushed into the FP64 sum every
+// 4 pairs.  Near pairs: FP64 r^2, the reference's clamp and strict cutoff
+// test, FP64 LJ with a Newton-refined reciprocal.
 template <int PPL, int MATH>
 __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                                           const double4 *__restrict__ L, int cap,
@@ -333,15 +381,30 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                                           const double (&px)[PPL], const double (&py)[PPL],
                                           const double (&pz)[PPL], double (&acc)[PPL]) {
     const double rc2 = J->rc2, min_r2 = J->min_r2;
+    const int capA = near_region_start<MATH>(cap);
+    const double4 *__restrict__ Ls = L + capA - 1;  // shell k at Ls[-k]
+    const double4 *__restrict__ Ln = L + capA;      // near k at Ln[k]
     for (int s = 0; s < nseg; ++s) {
-        const int t = uniform(seg[5 * s]);
-        const int fb = uniform(seg[5 * s + 1]), fe = uniform(seg[5 * s + 2]);
-        const int ub = uniform(seg[5 * s + 3]), ue = uniform(seg[5 * s + 4]);
+        const int t = uniform(seg[kSegInts * s]);
+        const int fb = uniform(seg[kSegInts * s + 1]), fe = uniform(seg[kSegInts * s + 2]);
+        const int sb = uniform(seg[kSegInts * s + 3]), se = uniform(seg[kSegInts * s + 4]);
+        const int nb = uniform(seg[kSegInts * s + 5]), ne = uniform(seg[kSegInts * s + 6]);
         const TypeParams P = J->types[t];
         if (MATH == MATH_COUNT) {
 #pragma unroll
             for (int q = 0; q < PPL; ++q) acc[q] += (double)(fe - fb);
-        } else if (MATH == MATH_MIXED) {
+            for (int k = sb; k < se; ++k) {
+                const double4 A = Ls[-k];
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) {
+                    const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
+                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    acc[q] += r2 < rc2 ? 1.0 : 0.0;
+                }
+            }
+            continue;
+        }
+        if (MATH == MATH_MIXED) {
             const float sig2f = P.sig2f, eps4f = P.eps4f;
             int r = fb;
             for (; r + 4 <= fe; r += 4) {
@@ -353,50 +416,83 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                     const double4 A = L[r + u];
 #pragma unroll
                     for (int q = 0; q < PPL; ++q) {
-                        const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
-                        const double r2 = dx * dx + dy * dy + dz * dz;
-                        const float inv = __builtin_amdgcn_rcpf((float)r2);
-                        const float s2 = sig2f * inv;
-                        const float s6 = s2 * s2 * s2;
-                        a32[q] = __builtin_fmaf(eps4f, __builtin_fmaf(s6, s6, -s6), a32[q]);
+                        double r2;
+                        a32[q] = __builtin_fmaf(eps4f, lj32(px[q], py[q], pz[q], A, sig2f, r2),
+                                                a32[q]);
                     }
                 }
 #pragma unroll
                 for (int q = 0; q < PPL; ++q) acc[q] += (double)a32[q];
             }
-            for (; r < fe; ++r) {
-                const double4 A = L[r];
+            if (r < fe) {
+                float a32[PPL];
 #pragma unroll
-                for (int q = 0; q < PPL; ++q) {
-                    const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
-                    const double r2 = dx * dx + dy * dy + dz * dz;
-                    const float inv = __builtin_amdgcn_rcpf((float)r2);
-                    const float s2 = sig2f * inv;
-                    const float s6 = s2 * s2 * s2;
-                    acc[q] += (double)(eps4f * __builtin_fmaf(s6, s6, -s6));
+                for (int q = 0; q < PPL; ++q) a32[q] = 0.0f;
+                for (; r < fe; ++r) {
+                    const double4 A = L[r];
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q) {
+                        double r2;
+                        a32[q] = __builtin_fmaf(eps4f, lj32(px[q], py[q], pz[q], A, sig2f, r2),
+                                                a32[q]);
+                    }
                 }
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) acc[q] += (double)a32[q];
+            }
+            int k = sb;
+            for (; k + 4 <= se; k += 4) {
+                float a32[PPL];
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) a32[q] = 0.0f;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) {
+                    const double4 A = Ls[-(k + u)];
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q) {
+                        double r2;
+                        const float e = lj32(px[q], py[q], pz[q], A, sig2f, r2);
+                        const float v = __builtin_fmaf(eps4f, e, a32[q]);
+                        a32[q] = r2 < rc2 ? v : a32[q];  // src/computegridsheet.h:79
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) acc[q] += (double)a32[q];
+            }
+            if (k < se) {
+                float a32[PPL];
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) a32[q] = 0.0f;
+                for (; k < se; ++k) {
+                    const double4 A = Ls[-k];
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q) {
+                        double r2;
+                        const float e = lj32(px[q], py[q], pz[q], A, sig2f, r2);
+                        const float v = __builtin_fmaf(eps4f, e, a32[q]);
+                        a32[q] = r2 < rc2 ? v : a32[q];
+                    }
+                }
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) acc[q] += (double)a32[q];
             }
         }
         const double sig2 = P.sig2, eps4 = P.eps4;
-        for (int k = ub; k < ue; ++k) {
-            const double4 A = L[cap - 1 - k];
+        for (int k = nb; k < ne; ++k) {
+            const double4 A = Ln[k];
 #pragma unroll
             for (int q = 0; q < PPL; ++q) {
                 const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
                 double r2 = dx * dx + dy * dy + dz * dz;
                 r2 = r2 > min_r2 ? r2 : min_r2;  // src/computegridsheet.h:77
                 const bool in = r2 < rc2;        // src/computegridsheet.h:79
-                if (MATH == MATH_COUNT) {
-                    acc[q] += in ? 1.0 : 0.0;
-                } else {
-                    double y = (double)__builtin_amdgcn_rcpf((float)r2);
-                    y = fma(fma(-r2, y, 1.0), y, y);
-                    if (MATH == MATH_FP64) y = fma(fma(-r2, y, 1.0), y, y);
-                    const double s2 = sig2 * y;
-                    const double s6 = s2 * s2 * s2;
-                    const double e = eps4 * fma(s6, s6, -s6);
-                    acc[q] += in ? e : 0.0;
-                }
+                double y = (double)__builtin_amdgcn_rcpf((float)r2);
+                y = fma(fma(-r2, y, 1.0), y, y);
+                if (MATH == MATH_FP64) y = fma(fma(-r2, y, 1.0), y, y);
+                const double s2 = sig2 * y;
+                const double s6 = s2 * s2 * s2;
+                const double e = eps4 * fma(s6, s6, -s6);
+                acc[q] += in ? e : 0.0;
             }
         }
     }
@@ -408,8 +504,8 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                                                unsigned long long *__restrict__ counter,
                                                int cap) {
     extern __shared__ double4 s_list[];
-    __shared__ int s_seg[5 * kMaxSeg];
-    __shared__ int s_cnt[2 * (kWG / 64)];
+    __shared__ int s_seg[kSegInts * kMaxSeg];
+    __shared__ int s_cnt[3 * (kWG / 64)];
     __shared__ long long s_item;
     const int tid = threadIdx.x;
 

@@ -44,7 +44,7 @@ struct Particle {
 // runs alone or inside a batch.
 struct Tuning {
     int ppl = 1;                 // points per lane (1 or 2)
-    int cap = 1536;              // LDS list capacity (records of 32 B)
+    int cap = 1024;              // LDS list capacity (records of 32 B)
     int tile[3] = {8, 8, 4};     // grid tile in points, product = 256 * ppl
     int widom_per_bin = 256;     // target insertions per Widom bin
     double s2_switch = 0.25;     // (sigma/r)^2 above which a pair takes the FP64 path

@@ -19,7 +19,11 @@ GOLDEN = os.path.join(HERE, "golden", "irmof1")
 
 def workdir(tmp_path, settings):
     d = tmp_path / "run"
-    shutil.copytree(GOLDEN, d)
+    shutil.copytree(GOLDEN, d, copy_function=shutil.copyfile)
+    for root, dirs, files in os.walk(d):  # the checkout may be read-only
+        os.chmod(root, 0o755)
+        for name in files:
+            os.chmod(os.path.join(root, name), 0o644)
     (d / "output_files").mkdir()
     (d / "data" / "grids").mkdir()
     text = (d / "simulation.input").read_text()

@@ -26,7 +26,11 @@ needs_ref = pytest.mark.skipif(not oracle.have_ref(), reason="oracle/_ref not bu
 
 def workdir(tmp_path):
     d = tmp_path / "run"
-    shutil.copytree(GOLDEN, d)
+    shutil.copytree(GOLDEN, d, copy_function=shutil.copyfile)
+    for root, dirs, files in os.walk(d):  # the checkout may be read-only
+        os.chmod(root, 0o755)
+        for name in files:
+            os.chmod(os.path.join(root, name), 0o644)
     (d / "output_files").mkdir()
     (d / "data" / "grids").mkdir()
     return d
