@@ -58,6 +58,28 @@ class Task(ctypes.Structure):
                 ("temperature", ctypes.c_double), ("seed", ctypes.c_uint64)]
 
 
+class ServerConfig(ctypes.Structure):
+    _fields_ = [("streams", ctypes.c_int), ("items_per_launch", ctypes.c_int64),
+                ("max_queue", ctypes.c_int)]
+
+
+class Request(ctypes.Structure):
+    _fields_ = [("atoms", ctypes.POINTER(Particle)), ("n_atoms", ctypes.c_int),
+                ("t_matrix", ctypes.c_double * 9), ("rep", ctypes.c_int * 3),
+                ("rc2", ctypes.c_double), ("grid_n", ctypes.c_int * 3),
+                ("precision", ctypes.c_int), ("n_insertions", ctypes.c_uint64),
+                ("temperature", ctypes.c_double), ("seed", ctypes.c_uint64)]
+
+
+class RequestStatus(ctypes.Structure):
+    _fields_ = [("state", ctypes.c_int), ("status", ctypes.c_int), ("t_submit", ctypes.c_double),
+                ("t_admit", ctypes.c_double), ("t_done", ctypes.c_double),
+                ("widom", WidomResult)]
+
+
+RQ_QUEUED, RQ_RUNNING, RQ_DONE, RQ_FAILED, RQ_REJECTED = range(5)
+
+
 class HostInfo(ctypes.Structure):
     _fields_ = [("a", ctypes.c_double), ("b", ctypes.c_double), ("c", ctypes.c_double),
                 ("alpha", ctypes.c_double), ("beta", ctypes.c_double), ("gamma", ctypes.c_double),
@@ -81,7 +103,9 @@ EXPORTS = [
     "er_grid_info", "er_grid_release", "er_widom", "er_count_incut_grid",
     "er_count_incut_widom", "er_widom_point", "er_henry_result", "er_batch_create", "er_batch_run",
     "er_batch_wait", "er_batch_result", "er_batch_release", "er_batch_set_timing",
-    "er_batch_kernel_ms", "er_host_setup",
+    "er_batch_kernel_ms", "er_grid_upload", "er_pockets_label", "er_widom_grid",
+    "er_server_create", "er_server_submit", "er_server_poll", "er_server_fetch_grid",
+    "er_server_release", "er_server_stats", "er_server_destroy", "er_host_setup",
     "er_host_run_info", "er_host_run_particles", "er_host_run_label",
     "er_host_run_write_settings", "er_host_run_release", "er_write_grid_file",
 ]
@@ -128,6 +152,20 @@ _SIGS = {
     "er_batch_set_timing": (ctypes.c_int, [_P, ctypes.c_int]),
     "er_batch_kernel_ms": (ctypes.c_int, [_P, ctypes.POINTER(ctypes.c_double),
                                           ctypes.POINTER(ctypes.c_int)]),
+    "er_grid_upload": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_P)]),
+    "er_pockets_label": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_int,
+                                        ctypes.POINTER(ctypes.c_int), _P]),
+    "er_widom_grid": (ctypes.c_int, [_P, _P, ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64,
+                                     ctypes.POINTER(WidomResult), _P]),
+    "er_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
+    "er_server_submit": (ctypes.c_int, [_P, ctypes.POINTER(Request),
+                                        ctypes.POINTER(ctypes.c_uint64)]),
+    "er_server_poll": (ctypes.c_int, [_P, ctypes.c_uint64, ctypes.c_double,
+                                      ctypes.POINTER(RequestStatus)]),
+    "er_server_fetch_grid": (ctypes.c_int, [_P, ctypes.c_uint64, _P]),
+    "er_server_release": (ctypes.c_int, [_P, ctypes.c_uint64]),
+    "er_server_stats": (ctypes.c_int, [_P] + [ctypes.POINTER(ctypes.c_int64)] * 4),
+    "er_server_destroy": (ctypes.c_int, [_P]),
     "er_host_setup": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                      ctypes.c_int, ctypes.POINTER(_P)]),
     "er_host_run_info": (ctypes.c_int, [_P, ctypes.POINTER(HostInfo)]),
@@ -341,6 +379,127 @@ class Batch:
     def close(self) -> None:
         if self.h:
             lib().er_batch_release(self.h)
+            self.h = None
+
+
+class Grid:
+    """A resident FP32/FP64 grid (from er_grid_upload or er_grid_compute)."""
+
+    def __init__(self, handle):
+        self.h = handle
+
+    @classmethod
+    def upload(cls, ctx: Context, grid: np.ndarray) -> "Grid":
+        g = np.ascontiguousarray(grid, dtype=np.float32)
+        h = _P()
+        _check(lib().er_grid_upload(ctx.h, g.ctypes.data_as(ctypes.c_void_p), _iarr(g.shape),
+                                    ctypes.byref(h)))
+        return cls(h)
+
+    @classmethod
+    def compute(cls, ctx: Context, fw: Framework, n: Sequence[int], precision: int = ER_FP32):
+        h = _P()
+        _check(lib().er_grid_compute(ctx.h, fw.h, _iarr(n), precision, None, ctypes.byref(h)))
+        return cls(h)
+
+    def download(self) -> np.ndarray:
+        n = (ctypes.c_int * 3)()
+        prec = ctypes.c_int()
+        _check(lib().er_grid_info(self.h, n, ctypes.byref(prec)))
+        out = np.empty(n[0] * n[1] * n[2], dtype=np.float64 if prec.value else np.float32)
+        _check(lib().er_grid_download(self.h, out.ctypes.data_as(ctypes.c_void_p)))
+        return out.reshape(n[0], n[1], n[2])
+
+    def close(self) -> None:
+        if self.h:
+            lib().er_grid_release(self.h)
+            self.h = None
+
+
+def pockets_label(ctx: Context, grid: Grid, temperature: float, block: bool = True,
+                  segments: bool = False):
+    """(n_pockets, segment ids or None); blocks pocket voxels in place if asked."""
+    npk = ctypes.c_int()
+    seg = None
+    if segments:
+        n = (ctypes.c_int * 3)()
+        _check(lib().er_grid_info(grid.h, n, None))
+        seg = np.empty((n[0], n[1], n[2]), dtype=np.int32)
+    _check(lib().er_pockets_label(ctx.h, grid.h, float(temperature), int(block),
+                                  ctypes.byref(npk),
+                                  seg.ctypes.data_as(ctypes.c_void_p) if segments else None))
+    return npk.value, seg
+
+
+def widom_grid(ctx: Context, grid: Grid, n: int, temperature: float, seed: int,
+               energies: bool = False):
+    res = WidomResult()
+    e = np.empty(int(n), dtype=np.float64) if energies else None
+    _check(lib().er_widom_grid(ctx.h, grid.h, int(n), float(temperature), int(seed),
+                               ctypes.byref(res),
+                               e.ctypes.data_as(ctypes.c_void_p) if energies else None))
+    return (res.sum_b, res.sum_eb, res.n, e) if energies else (res.sum_b, res.sum_eb, res.n)
+
+
+class Server:
+    """Continuous-batching server on one context (er_server_*)."""
+
+    def __init__(self, ctx: Context, streams: int = 3, items_per_launch: int = 4096,
+                 max_queue: int = 4096):
+        cfg = ServerConfig(streams, items_per_launch, max_queue)
+        h = _P()
+        _check(lib().er_server_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
+        self.h = h
+        self._keep = {}
+
+    def submit(self, frac, eps, sig2, t_matrix, rep, rc2, grid_n=(0, 0, 0),
+               precision=ER_FP32, n_insertions=0, temperature=298.0, seed=0):
+        """Returns (status code, ticket); codes 6 and 8 are rejections."""
+        arr, n = particles_array(np.asarray(frac, dtype=np.float64),
+                                 np.asarray(eps, dtype=np.float64),
+                                 np.asarray(sig2, dtype=np.float64))
+        rq = Request()
+        rq.atoms = ctypes.cast(arr, ctypes.POINTER(Particle))
+        rq.n_atoms = n
+        t = np.asarray(t_matrix, dtype=np.float64).reshape(9)
+        for i in range(9):
+            rq.t_matrix[i] = t[i]
+        for d in range(3):
+            rq.rep[d] = int(rep[d])
+            rq.grid_n[d] = int(grid_n[d])
+        rq.rc2 = float(rc2)
+        rq.precision = int(precision)
+        rq.n_insertions = int(n_insertions)
+        rq.temperature = float(temperature)
+        rq.seed = int(seed)
+        tk = ctypes.c_uint64()
+        code = lib().er_server_submit(self.h, ctypes.byref(rq), ctypes.byref(tk))
+        if code not in (0, 6, 8):
+            _check(code)
+        return code, tk.value
+
+    def poll(self, ticket: int, wait_ms: float = 0.0) -> RequestStatus:
+        st = RequestStatus()
+        _check(lib().er_server_poll(self.h, int(ticket), float(wait_ms), ctypes.byref(st)))
+        return st
+
+    def fetch_grid(self, ticket: int, shape, precision=ER_FP32) -> np.ndarray:
+        out = np.empty(tuple(shape), dtype=np.float64 if precision == ER_FP64 else np.float32)
+        _check(lib().er_server_fetch_grid(self.h, int(ticket),
+                                          out.ctypes.data_as(ctypes.c_void_p)))
+        return out
+
+    def release(self, ticket: int) -> None:
+        _check(lib().er_server_release(self.h, int(ticket)))
+
+    def stats(self):
+        v = [ctypes.c_int64() for _ in range(4)]
+        _check(lib().er_server_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("submitted", "rejected", "completed", "launches"), [x.value for x in v]))
+
+    def close(self) -> None:
+        if self.h:
+            lib().er_server_destroy(self.h)
             self.h = None
 
 

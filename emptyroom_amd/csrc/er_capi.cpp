@@ -7,6 +7,7 @@
 #include "../../include/emptyroom.h"
 #include "er_host.h"
 #include "er_runtime.h"
+#include "er_server.h"
 
 using er::ST_ARG;
 using er::ST_IO;
@@ -276,6 +277,133 @@ int er_batch_kernel_ms(er_batch *batch, double *ms, int *launches) {
 
 int er_batch_release(er_batch *batch) {
     delete B(batch);
+    return ST_OK;
+}
+
+int er_grid_upload(er_context *ctx, const float *host, const int n[3], er_grid **out) {
+    if (!ctx || !host || !n || !out) return arg_error("er_grid_upload: null argument");
+    return guarded([&]() -> int {
+        er::GridDev *g = nullptr;
+        int st = er::grid_upload(C(ctx), host, n, &g);
+        if (st == ST_OK) *out = reinterpret_cast<er_grid *>(g);
+        return st;
+    });
+}
+
+int er_pockets_label(er_context *ctx, er_grid *grid, double temperature, int block_in_place,
+                     int *n_pockets, int32_t *segments_out) {
+    if (!ctx || !grid || !n_pockets) return arg_error("er_pockets_label: null argument");
+    return guarded([&]() -> int {
+        return er::pockets_label(C(ctx), G(grid), temperature, block_in_place != 0, n_pockets,
+                                 segments_out);
+    });
+}
+
+int er_widom_grid(er_context *ctx, er_grid *grid, uint64_t n_insertions, double temperature,
+                  uint64_t seed, er_widom_result *out, double *energies_out) {
+    if (!ctx || !grid || !out) return arg_error("er_widom_grid: null argument");
+    return guarded([&]() -> int {
+        er::WidomResult r;
+        int st = er::widom_grid_run(C(ctx), G(grid), n_insertions, temperature, seed, &r,
+                                    energies_out);
+        if (st == ST_OK) {
+            out->sum_b = r.sum_b;
+            out->sum_eb = r.sum_eb;
+            out->n = r.n;
+        }
+        return st;
+    });
+}
+
+int er_server_create(er_context *ctx, const er_server_config *config, er_server **out) {
+    if (!ctx || !out) return arg_error("er_server_create: null argument");
+    return guarded([&]() -> int {
+        er::ServerConfig cfg;
+        if (config) {
+            cfg.streams = config->streams;
+            cfg.items_per_launch = config->items_per_launch;
+            cfg.max_queue = config->max_queue;
+        }
+        if (cfg.streams < 1 || cfg.streams > 8 || cfg.items_per_launch < 0 || cfg.max_queue < 1)
+            return arg_error("er_server_create: streams 1..8, items_per_launch >= 0, max_queue >= 1");
+        *out = reinterpret_cast<er_server *>(new er::Server(C(ctx), cfg));
+        return ST_OK;
+    });
+}
+
+int er_server_submit(er_server *server, const er_request *rq, uint64_t *ticket) {
+    if (!server || !rq || !ticket || (rq->n_atoms > 0 && !rq->atoms) || rq->n_atoms < 0)
+        return arg_error("er_server_submit: bad arguments");
+    const bool grid = rq->grid_n[0] || rq->grid_n[1] || rq->grid_n[2];
+    if (grid)
+        for (int d = 0; d < 3; d++)
+            if (rq->grid_n[d] < 2 || rq->grid_n[d] > 65536)
+                return arg_error("er_server_submit: grid dimensions must be in [2, 65536]");
+    if (grid && rq->precision != 0 && rq->precision != 1)
+        return arg_error("er_server_submit: precision must be 0 or 1");
+    if (rq->n_insertions && (rq->n_insertions >= (1ULL << 32) || !(rq->temperature > 0.0)))
+        return arg_error("er_server_submit: insertions in [1, 2^32) and positive temperature");
+    if (!grid && !rq->n_insertions) return arg_error("er_server_submit: empty request");
+    return guarded([&]() -> int {
+        er::RequestSpec spec;
+        spec.atoms.assign(reinterpret_cast<const er::Particle *>(rq->atoms),
+                          reinterpret_cast<const er::Particle *>(rq->atoms) + rq->n_atoms);
+        std::memcpy(spec.t, rq->t_matrix, sizeof(spec.t));
+        std::memcpy(spec.rep, rq->rep, sizeof(spec.rep));
+        spec.rc2 = rq->rc2;
+        std::memcpy(spec.grid_n, rq->grid_n, sizeof(spec.grid_n));
+        spec.precision = rq->precision;
+        spec.n_ins = rq->n_insertions;
+        spec.temperature = rq->temperature;
+        spec.seed = rq->seed;
+        return reinterpret_cast<er::Server *>(server)->submit(spec, ticket);
+    });
+}
+
+int er_server_poll(er_server *server, uint64_t ticket, double wait_ms, er_request_status *out) {
+    if (!server || !out) return arg_error("er_server_poll: null argument");
+    return guarded([&]() -> int {
+        er::RequestInfo info;
+        int st = reinterpret_cast<er::Server *>(server)->poll(ticket, &info, wait_ms);
+        if (st != ST_OK) return st;
+        out->state = info.state;
+        out->status = info.status;
+        out->t_submit = info.t_submit;
+        out->t_admit = info.t_admit;
+        out->t_done = info.t_done;
+        out->widom.sum_b = info.widom.sum_b;
+        out->widom.sum_eb = info.widom.sum_eb;
+        out->widom.n = info.widom.n;
+        return ST_OK;
+    });
+}
+
+int er_server_fetch_grid(er_server *server, uint64_t ticket, void *host_out) {
+    if (!server || !host_out) return arg_error("er_server_fetch_grid: null argument");
+    return guarded([&]() -> int {
+        return reinterpret_cast<er::Server *>(server)->fetch_grid(ticket, host_out);
+    });
+}
+
+int er_server_release(er_server *server, uint64_t ticket) {
+    if (!server) return arg_error("er_server_release: null server");
+    return guarded([&]() -> int { return reinterpret_cast<er::Server *>(server)->release(ticket); });
+}
+
+int er_server_stats(er_server *server, int64_t *submitted, int64_t *rejected, int64_t *completed,
+                    int64_t *launches) {
+    if (!server) return arg_error("er_server_stats: null server");
+    long long a = 0, b = 0, c = 0, d = 0;
+    reinterpret_cast<er::Server *>(server)->stats(&a, &b, &c, &d);
+    if (submitted) *submitted = a;
+    if (rejected) *rejected = b;
+    if (completed) *completed = c;
+    if (launches) *launches = d;
+    return ST_OK;
+}
+
+int er_server_destroy(er_server *server) {
+    delete reinterpret_cast<er::Server *>(server);
     return ST_OK;
 }
 

@@ -100,7 +100,8 @@ struct JobDev {
     double *e_out;      // FP64 energy per insertion index
     // scheduling
     long long item_begin;  // first global work item of this job
-    long long n_items;     // grid tiles or Widom bins
+    long long n_items;     // grid tiles or Widom bins of this job in this launch
+    long long local0;      // first tile / bin of the job covered by this launch
 };
 
 // Kernel launch geometry.

@@ -522,7 +522,7 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             else jh = mid - 1;
         }
         const JobDev *__restrict__ J = jobs + jl;
-        const long long local = item - J->item_begin;
+        const long long local = item - J->item_begin + J->local0;
         double lo[3], hi[3];
 
         if (J->kind == JOB_GRID) {

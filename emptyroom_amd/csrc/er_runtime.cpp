@@ -471,6 +471,16 @@ int run_jobs(Context *ctx, int math, JobDev *d_jobs, int njobs, long long total,
     return ST_OK;
 }
 
+struct Scoped {
+    Context *ctx;
+    std::vector<void *> ptrs;
+    ~Scoped() {
+        for (void *p : ptrs) ctx->pool.free(p);
+    }
+};
+
+}  // namespace
+
 int widom_prepare(Context *ctx, FrameworkDev *fw, uint64_t n_ins, WidomBuffers *wb,
                   std::vector<void *> *owned) {
     if (n_ins == 0 || n_ins >= (1ULL << 32)) {
@@ -498,15 +508,20 @@ int widom_enqueue_binning(Context *ctx, const WidomBuffers &wb, uint64_t n_ins, 
     return ST_OK;
 }
 
-struct Scoped {
-    Context *ctx;
-    std::vector<void *> ptrs;
-    ~Scoped() {
-        for (void *p : ptrs) ctx->pool.free(p);
-    }
-};
 
-}  // namespace
+size_t framework_near_capacity(int n_atoms, const double t[9], const int rep[3], double rc2) {
+    double inv[9];
+    invert3(t, inv);
+    const double rc = std::sqrt(rc2);
+    size_t per = 1;
+    for (int d = 0; d < 3; d++) {
+        const double ext = rc / perp_width(inv, d) + 1e-7;
+        const long long k =
+            std::min<long long>(2LL * rep[d] + 1, (long long)std::floor(1.0 + 2.0 * ext) + 1);
+        per *= (size_t)std::max<long long>(k, 1);
+    }
+    return std::max<size_t>(1, (size_t)std::max(n_atoms, 0) * per);
+}
 
 // ---------------------------------------------------------------------------
 // Plain path

@@ -145,6 +145,25 @@ JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void
                      const Tuning &tu, int math);
 JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
                       const WidomBuffers &wb);
+// Allocates the Widom buffers of a job (owned pointers appended to `owned`).
+int widom_prepare(Context *ctx, FrameworkDev *fw, uint64_t n_ins, WidomBuffers *wb,
+                  std::vector<void *> *owned);
+// Enqueues point generation, binning and the bin-sorted index list on s.
+int widom_enqueue_binning(Context *ctx, const WidomBuffers &wb, uint64_t n_ins, uint64_t seed,
+                          uint32_t stream_id, hipStream_t s);
+// Upper bound on the near-list records framework_upload allocates.
+size_t framework_near_capacity(int n_atoms, const double t[9], const int rep[3], double rc2);
+
+// Config 5: FP32 grids uploaded from the host, pocket blocking
+// (src/pocketblocking.cc:126-391) and grid-interpolated Widom (src/gcmc.cc:145-177).
+int grid_upload(Context *ctx, const float *host, const int n[3], GridDev **out);
+int pockets_label(Context *ctx, GridDev *g, double temperature, bool block, int *n_pockets,
+                  int32_t *seg_host);
+int widom_grid_run(Context *ctx, GridDev *g, uint64_t n_ins, double temperature, uint64_t seed,
+                   WidomResult *out, double *energies_host);
+// Segment classification from face links (3 ints each: from, to, axis 1..3,
+// from < 0 = no link); returns the number of pockets, -1 on bad links.
+int classify_pockets(int nseg, const int *links, long long nlinks, std::vector<uint8_t> *pocket);
 
 // Batched path: many (framework, grid, Widom) tasks in one persistent launch,
 // captured in a hipGraph on first run and replayed.

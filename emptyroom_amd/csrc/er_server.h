@@ -1,0 +1,99 @@
+// er_server.h -- continuous-batching server for grid and Henry requests.
+//
+// Requests arrive at any time (er_server_submit).  A scheduler thread admits
+// them in arrival order while the context's HBM cap allows (framework upload,
+// output buffers, Widom binning), then keeps up to `streams` launches of the
+// tiled energy kernel in flight; every launch takes pending work items
+// (grid tiles, Widom bins) from all admitted requests, oldest first, up to a
+// work budget, so requests that arrive while others run join the next launch
+// and their work overlaps on the device.  A request completes when all its
+// items have run (plus the Widom reduction); its memory is then released.
+// Results are bit-identical to the plain path: an item's result depends only
+// on its own tile and framework.
+#pragma once
+
+#include <stdint.h>
+
+#include <atomic>
+#include <condition_variable>
+#include <deque>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "er_runtime.h"
+
+namespace er {
+
+struct ServerConfig {
+    int streams = 3;                  // launches in flight
+    long long items_per_launch = 4096;  // work items per launch (0 = all pending)
+    int max_queue = 4096;             // waiting requests beyond which submit rejects
+};
+
+enum RequestState : int { RQ_QUEUED = 0, RQ_RUNNING = 1, RQ_DONE = 2, RQ_FAILED = 3,
+                          RQ_REJECTED = 4 };
+
+struct RequestSpec {
+    std::vector<Particle> atoms;
+    double t[9];
+    int rep[3];
+    double rc2;
+    int grid_n[3];      // all zero: no grid
+    int precision;      // grid precision (0 FP32, 1 FP64)
+    uint64_t n_ins;     // 0: no Widom
+    double temperature;
+    uint64_t seed;
+};
+
+struct RequestInfo {
+    int state = RQ_QUEUED;
+    int status = ST_OK;
+    std::string error;
+    double t_submit = 0, t_admit = 0, t_done = 0;  // seconds, steady clock
+    WidomResult widom;
+};
+
+class Server {
+  public:
+    Server(Context *ctx, const ServerConfig &cfg);
+    ~Server();
+    // Returns ST_OK and a ticket, or ST_CAPACITY if the queue is full, or
+    // ST_NOMEM if the request alone exceeds the HBM cap (both count as rejections).
+    int submit(const RequestSpec &spec, uint64_t *ticket);
+    // Snapshot of a request; wait_ms < 0 blocks until it is done or failed.
+    int poll(uint64_t ticket, RequestInfo *out, double wait_ms);
+    // Copies a finished request's grid to host memory.
+    int fetch_grid(uint64_t ticket, void *host);
+    // Forget a finished request (frees its grid).
+    int release(uint64_t ticket);
+    void stats(long long *submitted, long long *rejected, long long *completed,
+               long long *launches);
+
+  private:
+    struct Req;
+    struct Launch;
+    void run();
+    bool admit_one(Req *r);
+    void finish(Req *r);
+    size_t estimate_bytes(const RequestSpec &s) const;
+
+    Context *ctx_;
+    ServerConfig cfg_;
+    std::mutex mu_;
+    std::condition_variable cv_, done_cv_;
+    std::deque<Req *> queue_;
+    std::vector<Req *> active_;
+    std::map<uint64_t, std::unique_ptr<Req>> all_;
+    std::vector<hipStream_t> streams_;
+    std::vector<std::unique_ptr<Launch>> launches_;
+    uint64_t next_ticket_ = 1;
+    std::atomic<bool> stop_{false};
+    long long n_submitted_ = 0, n_rejected_ = 0, n_completed_ = 0, n_launches_ = 0;
+    std::thread worker_;
+};
+
+}  // namespace er

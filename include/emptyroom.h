@@ -173,6 +173,76 @@ int er_batch_release(er_batch *batch);
 int er_batch_set_timing(er_batch *batch, int on);
 int er_batch_kernel_ms(er_batch *batch, double *ms, int *launches);
 
+/* ---- pocket blocking and grid-interpolated Widom ---------------------------
+ * An FP32 grid (from er_grid_compute, a batch, or er_grid_upload) is
+ * labelled as src/pocketblocking.cc:126-391 does: voxels with E >= 15 T are
+ * inaccessible, accessible voxels form 6-connected segments without periodic
+ * images numbered in raster order of their first voxel, the faces x = N-1 ->
+ * x = 0 (likewise y, z) link segments, and a segment whose part of that
+ * graph holds a loop with non-zero net cell translation is a channel; every
+ * other segment is a pocket.  With block_in_place, pocket voxels are set to
+ * 1e13 (FindAndBlockPockets, src/pocketblocking.cc:373-382).
+ * segments_out (nullable) receives the segment id of every voxel (-1 if
+ * inaccessible), layout k + j*nz + i*ny*nz.
+ * er_widom_grid scores n insertions of the Philox stream (seed, 1) by
+ * trilinear interpolation of the grid (src/gcmc.cc:145-177 after the wrap
+ * of src/gcmc.cc:74-77; lower index clamped to N-2). */
+int er_grid_upload(er_context *ctx, const float *host, const int n[3], er_grid **out);
+int er_pockets_label(er_context *ctx, er_grid *grid, double temperature, int block_in_place,
+                     int *n_pockets, int32_t *segments_out);
+int er_widom_grid(er_context *ctx, er_grid *grid, uint64_t n_insertions, double temperature,
+                  uint64_t seed, er_widom_result *out, double *energies_out);
+
+/* ---- serving ------------------------------------------------------------------
+ * No reference counterpart.  A server owns a scheduler thread on one context:
+ * requests (a framework plus a grid and/or a Widom run) are admitted in
+ * arrival order while the context's HBM cap allows, and every launch of the
+ * tiled kernel takes pending tiles / bins from all admitted requests, so work
+ * of requests that arrive while others run overlaps on the device.  Results
+ * are bit-identical to the plain path.  While a server runs, use its context
+ * only through the server. */
+typedef struct er_server_config {
+    int streams;               /* launches in flight (1..8) */
+    int64_t items_per_launch;  /* tiles + bins per launch, 0 = all pending */
+    int max_queue;             /* waiting requests beyond which submit rejects */
+} er_server_config;
+
+typedef struct er_request {
+    const er_particle *atoms;
+    int n_atoms;
+    double t_matrix[9];
+    int rep[3];
+    double rc2;
+    int grid_n[3];            /* all zero: no grid */
+    int precision;
+    uint64_t n_insertions;    /* 0: no Widom */
+    double temperature;
+    uint64_t seed;
+} er_request;
+
+typedef enum er_request_state {
+    ER_RQ_QUEUED = 0, ER_RQ_RUNNING = 1, ER_RQ_DONE = 2, ER_RQ_FAILED = 3, ER_RQ_REJECTED = 4
+} er_request_state;
+
+typedef struct er_request_status {
+    int state;                /* er_request_state */
+    int status;               /* er_status of a failed or rejected request */
+    double t_submit, t_admit, t_done;  /* seconds on the server's steady clock */
+    er_widom_result widom;
+} er_request_status;
+
+int er_server_create(er_context *ctx, const er_server_config *config, er_server **out);
+/* ER_E_CAPACITY (queue full) and ER_E_NOMEM (request alone exceeds the HBM
+ * cap) are rejections: the ticket is still valid and reports REJECTED. */
+int er_server_submit(er_server *server, const er_request *request, uint64_t *ticket);
+/* wait_ms < 0 blocks until the request is done, failed or rejected. */
+int er_server_poll(er_server *server, uint64_t ticket, double wait_ms, er_request_status *out);
+int er_server_fetch_grid(er_server *server, uint64_t ticket, void *host_out);
+int er_server_release(er_server *server, uint64_t ticket);
+int er_server_stats(er_server *server, int64_t *submitted, int64_t *rejected, int64_t *completed,
+                    int64_t *launches);
+int er_server_destroy(er_server *server);
+
 /* ---- host side of the drop-in binaries -------------------------------------
  * Everything src/writegrid.cu:64-127 / src/henry.cu:50-95 do before the
  * device work, from a directory laid out like the reference (simulation.input,
