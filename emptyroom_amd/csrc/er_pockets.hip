@@ -37,6 +37,18 @@ __device__ __forceinline__ int find_root(int *parent, int v) {
     return v;
 }
 
+// Root of v without path compression.  The flatten pass must not compress:
+// a halving store of another thread could land after v's own thread has
+// stored v's root and replace it with a stale (non-root) ancestor.
+__device__ __forceinline__ int find_root_ro(const int *parent, int v) {
+    int p = ld(&parent[v]);
+    while (p != v) {
+        v = p;
+        p = ld(&parent[v]);
+    }
+    return v;
+}
+
 __device__ __forceinline__ void unite(int *parent, int a, int b) {
     int ra = find_root(parent, a), rb = find_root(parent, b);
     while (ra != rb) {
@@ -82,7 +94,8 @@ __global__ void k_cc_flatten(long long nv, int *__restrict__ parent,
             is_root[v] = 0;
             continue;
         }
-        const int r = find_root(parent, (int)v);
+        // Only v's own thread writes parent[v] here, so it ends as the root.
+        const int r = find_root_ro(parent, (int)v);
         st(&parent[v], r);
         is_root[v] = (r == (int)v) ? 1u : 0u;
     }
@@ -148,6 +161,10 @@ __global__ void k_cc_block(long long nv, const int *__restrict__ seg,
 // with the lower index clamped to N-2 (f = 1 reads node N-1).
 __device__ __forceinline__ double trilinear(const float *__restrict__ g, int nx, int ny, int nz,
                                             double xf, double yf, double zf) {
+    // Separate multiplies and adds, as the reference's host code computes
+    // them: next to blocked (1e13 K) nodes a fused multiply-add changes the
+    // interpolated energy far beyond rounding of the result.
+#pragma clang fp contract(off)
     const double dx = 1.0 / (nx - 1), dy = 1.0 / (ny - 1), dz = 1.0 / (nz - 1);
     int ix = (int)floor(xf / dx), iy = (int)floor(yf / dy), iz = (int)floor(zf / dz);
     ix = ix > nx - 2 ? nx - 2 : (ix < 0 ? 0 : ix);
