@@ -109,6 +109,140 @@ def cpu_baseline(fws, uploaded, ctx, batch, budget_s, threads):
                 sample=sample, seconds=round(secs, 2)), grid_err, kh_err, e_err
 
 
+def percentile(v, q):
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def bench_serving(args, api, synth):
+    """Poisson arrivals of grid / Henry requests into the continuous-batching
+    server (HBM cap 200 GB); latency = completion - submission on the
+    server's clock; grids stay resident in HBM and are released unread."""
+    import threading
+    pool = [synth.config3_framework(i, config=4) for i in range(args.frameworks)]
+    parts = [(f, f.particles()) for f in pool]
+    ctx = api.Context(0, hbm_cap=200 << 30)
+    rates = [float(r) for r in args.rates.split(",")]
+    rows = []
+    for rate in rates:
+        n_req = max(50, int(rate * args.serve_seconds))
+        arrivals, kinds, temps = synth.config4_stream(rate, n_req)
+        srv = api.Server(ctx, streams=args.streams, items_per_launch=args.items_per_launch)
+        tickets = []
+        codes = []
+
+        def submit_all():
+            t0 = time.perf_counter()
+            for i in range(n_req):
+                dt = t0 + arrivals[i] - time.perf_counter()
+                if dt > 0:
+                    time.sleep(dt)
+                f, a = parts[i % len(parts)]
+                if kinds[i] == 0:
+                    code, tk = srv.submit(a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2,
+                                          grid_n=f.grid_n(), precision=api.ER_FP32)
+                else:
+                    code, tk = srv.submit(a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2,
+                                          n_insertions=f.n_insertions,
+                                          temperature=float(temps[i]), seed=f.seed + i)
+                codes.append(code)
+                tickets.append(tk)
+
+        th = threading.Thread(target=submit_all)
+        th.start()
+        th.join()
+        lat, t_first, t_last, done = [], None, None, 0
+        for code, tk in zip(codes, tickets):
+            if code != 0:
+                continue
+            st = srv.poll(tk, wait_ms=-1)
+            if st.state == api.RQ_DONE:
+                done += 1
+                lat.append((st.t_done - st.t_submit) * 1e3)
+                t_first = st.t_submit if t_first is None else min(t_first, st.t_submit)
+                t_last = st.t_done if t_last is None else max(t_last, st.t_done)
+            srv.release(tk)
+        stats = srv.stats()
+        srv.close()
+        span = (t_last - t_first) if done else float("nan")
+        row = dict(rate=rate, requests=n_req, completed=done,
+                   rejected=stats["rejected"], rejection_rate=stats["rejected"] / n_req,
+                   p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99),
+                   sustained_fws_per_s=done / span if done else 0.0,
+                   launches=stats["launches"])
+        log(f"serving rate {rate:g}/s: {json.dumps(row)}")
+        rows.append(row)
+    ctx.close()
+    top = rows[-1]
+    return {
+        "metric": "serving: sustained frameworks/s under Poisson arrivals (grid + Henry mix)",
+        "value": top["sustained_fws_per_s"], "unit": "frameworks/s", "n_gpus": 1,
+        "steps": len(rates), "warmup": 0, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 pair LJ beyond 2 sigma, fp64 r^2 / close pairs / accumulation",
+        "data": "synthetic (seeded SplitMix64 frameworks, Poisson arrivals)",
+        "config": {"workload": "config4: Poisson arrivals of grid / Henry requests, frameworks "
+                               "from the batching mix, T ~ U[250,350] K, HBM cap 200 GB",
+                   "frameworks_in_pool": len(pool), "streams": args.streams,
+                   "items_per_launch": args.items_per_launch,
+                   "seconds_per_rate": args.serve_seconds, "parallelism": "single GPU"},
+        "rates": rows,
+    }
+
+
+def bench_pockets(args, api, synth):
+    """Config 5: FP32 grids at 0.2 A, pocket blocking at 15 T, then 1e7
+    grid-interpolated insertions per framework (blocked voxels are 1e13 K)."""
+    T = 298.0
+    fws = synth.config5(args.frameworks)
+    ctx = api.Context(0)
+    grids, fwh = [], []
+    t0 = time.perf_counter()
+    for f in fws:
+        a = f.particles()
+        fw = api.Framework(ctx, a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2)
+        grids.append(api.Grid.compute(ctx, fw, f.grid_n(), api.ER_FP32))
+        fwh.append(fw)
+    t_grid = time.perf_counter() - t0
+    voxels = sum(int(np.prod(f.grid_n())) for f in fws)
+    log(f"{len(fws)} grids, {voxels:.3g} voxels in {t_grid:.2f} s")
+    t0 = time.perf_counter()
+    pockets = [api.pockets_label(ctx, g, T, block=True)[0] for g in grids]
+    t_lab = time.perf_counter() - t0
+    ins = 0
+    kh = []
+    t0 = time.perf_counter()
+    for i, (f, g) in enumerate(zip(fws, grids)):
+        sb, seb, n = api.widom_grid(ctx, g, f.n_insertions, T, f.seed)
+        ins += n
+        kh.append(sb / n)
+    t_tri = time.perf_counter() - t0
+    for g in grids:
+        g.close()
+    for fw in fwh:
+        fw.close()
+    ctx.close()
+    lab_gbs = 8.0 * voxels / t_lab / 1e9
+    tri_gbs = 32.0 * ins / t_tri / 1e9
+    log(f"labelling {t_lab * 1e3:.1f} ms ({lab_gbs:.1f} GB/s at 8 B/voxel), "
+        f"trilinear {ins / t_tri:.3g} inserts/s")
+    return {
+        "metric": "config5: pocket labelling voxels/s + grid-interpolated Widom inserts/s",
+        "value": ins / t_tri, "unit": "inserts/s", "n_gpus": 1, "steps": 1, "warmup": 0,
+        "ms_per_step": (t_lab + t_tri) * 1e3, "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": "fp32 grid, fp64 interpolation and Boltzmann sums",
+        "data": "synthetic (seeded SplitMix64 frameworks)",
+        "config": {"workload": f"config5: {len(fws)} frameworks, FP32 grids at 0.2 A, "
+                               "threshold 15 T at 298 K, 1e7 trilinear insertions each",
+                   "voxels": voxels, "insertions": ins, "parallelism": "single GPU"},
+        "labelling_voxels_per_s": voxels / t_lab, "labelling_ms": t_lab * 1e3,
+        "pockets_total": int(sum(pockets)), "grid_build_s": t_grid,
+        "roofline": {"bound": "hbm", "achieved": lab_gbs, "peak": 8000.0, "unit": "GB/s",
+                     "frac": lab_gbs / 8000.0, "traffic": None,
+                     "kernel": "pocket labelling (whole call, 8 B/voxel)"},
+        "trilinear_GBps_at_32B": tri_gbs,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -120,15 +254,54 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--graph", type=int, default=1, help="also time hipGraph replays")
+    ap.add_argument("--ppl", type=int, default=0, help="points per lane (0: library default)")
+    ap.add_argument("--tile", default="", help="grid tile in points, e.g. 8,8,8")
+    ap.add_argument("--lds-records", type=int, default=0)
+    ap.add_argument("--region-len", type=float, default=-1.0, help="region edge in A (-1: default)")
+    ap.add_argument("--s2-switch", type=float, default=0.0)
+    ap.add_argument("--widom-per-bin", type=int, default=0)
+    ap.add_argument("--rates", default="50,100,200,500", help="config4 arrival rates (req/s)")
+    ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--streams", type=int, default=3)
+    ap.add_argument("--items-per-launch", type=int, default=32768)
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
-    from emptyroom_amd import api
+    from emptyroom_amd import api, synth
+    if args.workload == "config4":
+        if args.frameworks == 256:
+            args.frameworks = 64
+        print(json.dumps(bench_serving(args, api, synth)))
+        return
+    if args.workload == "config5":
+        if args.frameworks == 256:
+            args.frameworks = 64
+        print(json.dumps(bench_pockets(args, api, synth)))
+        return
     fws = make_workload(args.workload, args.frameworks)
     log(f"workload {args.workload}: {len(fws)} frameworks")
     ctx = api.Context(0)
+    tu = ctx.tuning
+    if args.ppl:
+        tu.points_per_lane = args.ppl
+    if args.tile:
+        for d, v in enumerate(args.tile.split(",")):
+            tu.tile[d] = int(v)
+    if args.lds_records:
+        tu.lds_records = args.lds_records
+    if args.region_len >= 0:
+        tu.region_len = args.region_len
+    if args.s2_switch:
+        tu.s2_switch = args.s2_switch
+    if args.widom_per_bin:
+        tu.widom_per_bin = args.widom_per_bin
+    ctx.tuning = tu
+    tuning = dict(ppl=tu.points_per_lane, tile=list(tu.tile), lds_records=tu.lds_records,
+                  region_len=tu.region_len, s2_switch=tu.s2_switch,
+                  widom_per_bin=tu.widom_per_bin)
+    log(f"tuning {tuning}")
     uploaded = []
     for f in fws:
         a = f.particles()
@@ -229,8 +402,9 @@ def main():
         "roofline": {"bound": "mfma", "engine": "VALU FP32 (no MFMA: pairwise LJ)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
-                     "kernel": "k_tiles<1,MIXED>", "kernel_ms": kern_s * 1e3,
+                     "kernel": f"k_tiles<{tuning['ppl']},MIXED>", "kernel_ms": kern_s * 1e3,
                      "flop_per_pair": FLOP_PER_PAIR},
+        "tuning": tuning,
         "cpu_baseline": cpu,
     }
     print(json.dumps(out))

@@ -49,7 +49,7 @@ class WidomResult(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("points_per_lane", ctypes.c_int), ("lds_records", ctypes.c_int),
                 ("tile", ctypes.c_int * 3), ("widom_per_bin", ctypes.c_int),
-                ("s2_switch", ctypes.c_double)]
+                ("s2_switch", ctypes.c_double), ("region_len", ctypes.c_double)]
 
 
 class Task(ctypes.Structure):
@@ -444,7 +444,7 @@ def widom_grid(ctx: Context, grid: Grid, n: int, temperature: float, seed: int,
 class Server:
     """Continuous-batching server on one context (er_server_*)."""
 
-    def __init__(self, ctx: Context, streams: int = 3, items_per_launch: int = 4096,
+    def __init__(self, ctx: Context, streams: int = 3, items_per_launch: int = 32768,
                  max_queue: int = 4096):
         cfg = ServerConfig(streams, items_per_launch, max_queue)
         h = _P()

@@ -2,6 +2,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstring>
+#include <memory>
 #include <new>
 
 #include "../../include/emptyroom.h"
@@ -77,6 +78,7 @@ int er_context_get_tuning(const er_context *ctx, er_tuning *out) {
     for (int d = 0; d < 3; d++) out->tile[d] = t.tile[d];
     out->widom_per_bin = t.widom_per_bin;
     out->s2_switch = t.s2_switch;
+    out->region_len = t.region_len;
     return ST_OK;
 }
 
@@ -92,13 +94,20 @@ int er_context_set_tuning(er_context *ctx, const er_tuning *in) {
     if (in->widom_per_bin < 1) return arg_error("er_context_set_tuning: widom_per_bin < 1");
     if (!(in->s2_switch > 0.0) || !(in->s2_switch <= 1.0))
         return arg_error("er_context_set_tuning: s2_switch must be in (0, 1]");
-    er::Tuning &t = C(ctx)->tuning;
+    if (!(in->region_len >= 0.0) || !(in->region_len <= 1000.0))
+        return arg_error("er_context_set_tuning: region_len must be in [0, 1000] Angstrom");
+    er::Context *c = C(ctx);
+    er::Tuning &t = c->tuning;
     t.ppl = in->points_per_lane;
     t.cap = in->lds_records;
     for (int d = 0; d < 3; d++) t.tile[d] = in->tile[d];
     t.widom_per_bin = in->widom_per_bin;
     t.s2_switch = in->s2_switch;
-    return ST_OK;
+    t.region_len = in->region_len;
+    // a tuning with higher occupancy needs more region scratch slices
+    return guarded([&]() -> int {
+        return er::scratch_ensure(c, c->stream, &c->d_scratch, &c->scratch_wgs);
+    });
 }
 
 int er_context_memory(const er_context *ctx, size_t *in_use, size_t *reserved) {
@@ -326,7 +335,12 @@ int er_server_create(er_context *ctx, const er_server_config *config, er_server 
         }
         if (cfg.streams < 1 || cfg.streams > 8 || cfg.items_per_launch < 0 || cfg.max_queue < 1)
             return arg_error("er_server_create: streams 1..8, items_per_launch >= 0, max_queue >= 1");
-        *out = reinterpret_cast<er_server *>(new er::Server(C(ctx), cfg));
+        std::unique_ptr<er::Server> srv(new er::Server(C(ctx), cfg));
+        if (srv->status() != ST_OK) {
+            er::set_error("er_server_create: no device memory for the launch slots");
+            return srv->status();
+        }
+        *out = reinterpret_cast<er_server *>(srv.release());
         return ST_OK;
     });
 }

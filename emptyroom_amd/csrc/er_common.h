@@ -60,8 +60,7 @@ struct alignas(16) TypeParams {
     double sig2;
     double eps4;
     double rclose;
-    float sig2f;
-    float eps4f;
+    double pad;
 };
 
 // Work kinds of the tiled energy kernel.
@@ -88,6 +87,8 @@ struct JobDev {
     int n[3];           // grid points per axis, endpoints included
     int tile[3];        // tile extent in points
     int ntiles[3];      // tiles per axis
+    int region[3];      // tiles per region (work item) per axis
+    int nregions[3];    // regions per axis
     void *out;          // float*, double* or uint32_t* by mode; index k + j*nz + i*ny*nz
     // Widom
     int nbins[3];       // insertion bins per fractional axis
@@ -107,5 +108,8 @@ struct JobDev {
 // Kernel launch geometry.
 constexpr int kWG = 256;      // threads per workgroup (4 wave64)
 constexpr int kMaxSeg = 64;   // list segments per fill pass
+constexpr int kMaxTypes = 64; // pair types a region sub-list can index
+constexpr int kRegionRecords = 8192;  // records of one workgroup's region sub-list (256 KiB)
+constexpr long long kMinGridItems = 2048;  // regions per grid before regions shrink
 
 }  // namespace er

@@ -258,10 +258,13 @@ __device__ __forceinline__ int near_region_start(int cap) {
 // distance d to the tile centre: far (every point of the tile is inside the
 // cutoff and beyond the FP64 switch radius), shell (beyond the switch radius
 // but possibly outside the cutoff), near (possibly inside the switch radius).
-// Records keep the near-list order, so the list (and every sum over it)
-// depends only on the tile.  Returns the number of segments written to seg.
+// Records hold the atom relative to the tile centre and its squared distance,
+// (a - c, |a - c|^2), and keep the source order, so the list (and every sum
+// over it) depends only on the tile and the source.  Returns the number of
+// segments written to seg.
 template <int MATH>
-__device__ int fill_list(const JobDev *__restrict__ J, const Tile &tl, int &cur_t, int &cur_pos,
+__device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict__ src,
+                         const int *__restrict__ tbeg, const Tile &tl, int &cur_t, int &cur_pos,
                          int cap, double4 *__restrict__ L, int *__restrict__ seg,
                          int *__restrict__ s_cnt) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -286,7 +289,7 @@ __device__ int fill_list(const JobDev *__restrict__ J, const Tile &tl, int &cur_
             near2 = rn * rn;
             near_t = cur_t;
         }
-        const int tend = uniform(J->type_begin[cur_t + 1]);
+        const int tend = uniform(tbeg[cur_t + 1]);
         if (cur_pos >= tend) {
             if (nf > f0 || ns > s0 || nn > n0) {
                 if (tid == 0) {
@@ -299,7 +302,7 @@ __device__ int fill_list(const JobDev *__restrict__ J, const Tile &tl, int &cur_
             s0 = ns;
             n0 = nn;
             ++cur_t;
-            if (cur_t < ntypes) cur_pos = uniform(J->type_begin[cur_t]);
+            if (cur_t < ntypes) cur_pos = uniform(tbeg[cur_t]);
             continue;
         }
         if (nf + ns + kWG > capA || nn + kWG > capN || nseg >= kMaxSeg) break;
@@ -307,9 +310,10 @@ __device__ int fill_list(const JobDev *__restrict__ J, const Tile &tl, int &cur_
         int cat = CAT_NONE;
         double4 A = make_double4(0.0, 0.0, 0.0, 0.0);
         if (idx < tend) {
-            A = J->near[idx];
-            const double dx = A.x - tl.cx, dy = A.y - tl.cy, dz = A.z - tl.cz;
+            const double4 S = src[idx];
+            const double dx = S.x - tl.cx, dy = S.y - tl.cy, dz = S.z - tl.cz;
             const double d2 = dx * dx + dy * dy + dz * dz;
+            A = make_double4(dx, dy, dz, d2);
             if (d2 < in2) {
                 if (MATH == MATH_FP64) cat = CAT_NEAR;
                 else if (MATH == MATH_MIXED && d2 <= near2) cat = CAT_NEAR;
@@ -340,7 +344,7 @@ __device__ int fill_list(const JobDev *__restrict__ J, const Tile &tl, int &cur_
         }
         if (cat == CAT_FAR) L[of + lanes_below(bf)] = A;
         if (cat == CAT_SHELL) L[capA - 1 - (os + lanes_below(bs))] = A;
-        if (cat == CAT_NEAR) L[capA + on + lanes_below(bn)] = A;
+        if (cat == CAT_NEAR) L[capA + on + lanes_below(bn)] = A;  // A: (a - c, |a - c|^2)
         nf = uniform(nf + tf);
         ns = uniform(ns + ts);
         nn = uniform(nn + tn);
@@ -357,23 +361,18 @@ __device__ int fill_list(const JobDev *__restrict__ J, const Tile &tl, int &cur_
     return nseg;
 }
 
-// FP32 LJ of one far/shell pair: FP64 displacement and r^2, FP32 reciprocal,
-// sigma^2/r^2, s^6 and s^6 (s^6 - 1); returns the pair energy / (4 eps).
-__device__ __forceinline__ float lj32(double px, double py, double pz, const double4 &A,
-                                      float sig2f, double &r2) {
-    const double dx = px - A.x, dy = py - A.y, dz = pz - A.z;
-    r2 = dx * dx + dy * dy + dz * dz;
-    const float inv = __builtin_amdgcn_rcpf((float)r2);
-    const float s2 = sig2f * inv;
-    const float s6 = s2 * s2 * s2;
-    return __builtin_fmaf(s6, s6, -s6);
-}
-
 // Energies (or in-cutoff counts) of this lane's PPL points against the LDS
-// list.  Far pairs: FP32 LJ, no cutoff test.  Shell pairs: FP32 LJ behind the
-// FP64 strict cutoff test.  FP32 partials are flushed into the FP64 sum every
-// 4 pairs.  Near pairs: FP64 r^2, the reference's clamp and strict cutoff
-// test, FP64 LJ with a Newton-refined reciprocal.
+// list.  Points and records are relative to the tile centre c.
+//   Far pairs (inside r_c and beyond the FP64 switch radius for every point
+//   of the tile): r^2 = |a|^2 + |p|^2 - 2 p.a in FP64 (three FMAs and an add;
+//   |a| <= r_c + R keeps the cancellation below 13x), then in FP32
+//   u = 1/r^2, and per point the sums S3 = sum u^3 and S6 = sum u^6 over at
+//   most 8 pairs of one type, flushed into the FP64 energy as
+//   4 eps (sigma^12 S6 - sigma^6 S3).  sigma and eps stay FP64, so their
+//   rounding adds no error common to every pair of a type.
+//   Shell pairs: the same, behind the strict FP64 test r^2 < r_c^2.
+//   Near pairs: FP64 displacement and r^2, the reference's clamp and strict
+//   cutoff test, FP64 LJ with a Newton-refined reciprocal.
 template <int PPL, int MATH>
 __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                                           const double4 *__restrict__ L, int cap,
@@ -384,6 +383,14 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
     const int capA = near_region_start<MATH>(cap);
     const double4 *__restrict__ Ls = L + capA - 1;  // shell k at Ls[-k]
     const double4 *__restrict__ Ln = L + capA;      // near k at Ln[k]
+    double mx[PPL], my[PPL], mz[PPL], p2[PPL];
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+        mx[q] = -2.0 * px[q];
+        my[q] = -2.0 * py[q];
+        mz[q] = -2.0 * pz[q];
+        p2[q] = px[q] * px[q] + py[q] * py[q] + pz[q] * pz[q];
+    }
     for (int s = 0; s < nseg; ++s) {
         const int t = uniform(seg[kSegInts * s]);
         const int fb = uniform(seg[kSegInts * s + 1]), fe = uniform(seg[kSegInts * s + 2]);
@@ -397,84 +404,73 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                 const double4 A = Ls[-k];
 #pragma unroll
                 for (int q = 0; q < PPL; ++q) {
-                    const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
-                    const double r2 = dx * dx + dy * dy + dz * dz;
+                    const double r2 = fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
                     acc[q] += r2 < rc2 ? 1.0 : 0.0;
                 }
             }
             continue;
         }
         if (MATH == MATH_MIXED) {
-            const float sig2f = P.sig2f, eps4f = P.eps4f;
-            int r = fb;
-            for (; r + 4 <= fe; r += 4) {
-                float a32[PPL];
+            const double sig6 = P.sig2 * P.sig2 * P.sig2;
+            const double c6 = P.eps4 * sig6 * sig6, c3 = -P.eps4 * sig6;
+            // far records [fb, fe), then shell records [sb, se), 8 pairs per flush
+            for (int r = fb; r < fe; r += 8) {
+                float a3[PPL], a6[PPL];
 #pragma unroll
-                for (int q = 0; q < PPL; ++q) a32[q] = 0.0f;
+                for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
+                if (r + 8 <= fe) {
 #pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const double4 A = L[r + u];
+                    for (int u = 0; u < 8; ++u) {
+                        const double4 A = L[r + u];
 #pragma unroll
-                    for (int q = 0; q < PPL; ++q) {
-                        double r2;
-                        a32[q] = __builtin_fmaf(eps4f, lj32(px[q], py[q], pz[q], A, sig2f, r2),
-                                                a32[q]);
+                        for (int q = 0; q < PPL; ++q) {
+                            const double r2 =
+                                fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
+                            const float w = __builtin_amdgcn_rcpf((float)r2);
+                            const float w3 = w * w * w;
+                            a3[q] += w3;
+                            a6[q] = __builtin_fmaf(w3, w3, a6[q]);
+                        }
+                    }
+                } else {
+                    for (int k = r; k < fe; ++k) {
+                        const double4 A = L[k];
+#pragma unroll
+                        for (int q = 0; q < PPL; ++q) {
+                            const double r2 =
+                                fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
+                            const float w = __builtin_amdgcn_rcpf((float)r2);
+                            const float w3 = w * w * w;
+                            a3[q] += w3;
+                            a6[q] = __builtin_fmaf(w3, w3, a6[q]);
+                        }
                     }
                 }
 #pragma unroll
-                for (int q = 0; q < PPL; ++q) acc[q] += (double)a32[q];
+                for (int q = 0; q < PPL; ++q)
+                    acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
             }
-            if (r < fe) {
-                float a32[PPL];
+            for (int k0 = sb; k0 < se; k0 += 8) {
+                float a3[PPL], a6[PPL];
 #pragma unroll
-                for (int q = 0; q < PPL; ++q) a32[q] = 0.0f;
-                for (; r < fe; ++r) {
-                    const double4 A = L[r];
-#pragma unroll
-                    for (int q = 0; q < PPL; ++q) {
-                        double r2;
-                        a32[q] = __builtin_fmaf(eps4f, lj32(px[q], py[q], pz[q], A, sig2f, r2),
-                                                a32[q]);
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < PPL; ++q) acc[q] += (double)a32[q];
-            }
-            int k = sb;
-            for (; k + 4 <= se; k += 4) {
-                float a32[PPL];
-#pragma unroll
-                for (int q = 0; q < PPL; ++q) a32[q] = 0.0f;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) {
-                    const double4 A = Ls[-(k + u)];
-#pragma unroll
-                    for (int q = 0; q < PPL; ++q) {
-                        double r2;
-                        const float e = lj32(px[q], py[q], pz[q], A, sig2f, r2);
-                        const float v = __builtin_fmaf(eps4f, e, a32[q]);
-                        a32[q] = r2 < rc2 ? v : a32[q];  // src/computegridsheet.h:79
-                    }
-                }
-#pragma unroll
-                for (int q = 0; q < PPL; ++q) acc[q] += (double)a32[q];
-            }
-            if (k < se) {
-                float a32[PPL];
-#pragma unroll
-                for (int q = 0; q < PPL; ++q) a32[q] = 0.0f;
-                for (; k < se; ++k) {
+                for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
+                const int k1 = k0 + 8 < se ? k0 + 8 : se;
+                for (int k = k0; k < k1; ++k) {
                     const double4 A = Ls[-k];
 #pragma unroll
                     for (int q = 0; q < PPL; ++q) {
-                        double r2;
-                        const float e = lj32(px[q], py[q], pz[q], A, sig2f, r2);
-                        const float v = __builtin_fmaf(eps4f, e, a32[q]);
-                        a32[q] = r2 < rc2 ? v : a32[q];
+                        const double r2 =
+                            fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
+                        const float w = __builtin_amdgcn_rcpf((float)r2);
+                        float w3 = w * w * w;
+                        w3 = r2 < rc2 ? w3 : 0.0f;  // src/computegridsheet.h:79
+                        a3[q] += w3;
+                        a6[q] = __builtin_fmaf(w3, w3, a6[q]);
                     }
                 }
 #pragma unroll
-                for (int q = 0; q < PPL; ++q) acc[q] += (double)a32[q];
+                for (int q = 0; q < PPL; ++q)
+                    acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
             }
         }
         const double sig2 = P.sig2, eps4 = P.eps4;
@@ -498,16 +494,132 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
     }
 }
 
+// Energies of one grid tile (tile coordinates tc), candidates streamed from
+// src / tbeg (the framework's near list, or a region's sub-list of it).
+template <int PPL, int MATH>
+__device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
+                          const double4 *__restrict__ src, const int *__restrict__ tbeg, int cap,
+                          double4 *__restrict__ s_list, int *__restrict__ s_seg,
+                          int *__restrict__ s_cnt) {
+    const int tid = threadIdx.x;
+    int i0[3], i1[3];
+    double lo[3], hi[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        i0[d] = tc[d] * J->tile[d];
+        const int e = i0[d] + J->tile[d];
+        i1[d] = (e < J->n[d] ? e : J->n[d]) - 1;
+        // src/writegrid.cu:122-127
+        lo[d] = 1.0 * i0[d] / (J->n[d] - 1);
+        hi[d] = 1.0 * i1[d] / (J->n[d] - 1);
+    }
+    const Tile tl = tile_sphere(J->t, lo, hi);
+    double px[PPL], py[PPL], pz[PPL], acc[PPL];
+    long long oidx[PPL];
+    bool ok[PPL];
+    const int t1 = J->tile[1], t2 = J->tile[2];
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+        const int p = q * kWG + tid;
+        int gi = i0[0] + p / (t1 * t2);
+        int gj = i0[1] + (p / t2) % t1;
+        int gk = i0[2] + p % t2;
+        ok[q] = gi <= i1[0] && gj <= i1[1] && gk <= i1[2];
+        if (!ok[q]) {
+            gi = i0[0];
+            gj = i0[1];
+            gk = i0[2];
+        }
+        frac_to_cart(J->t, 1.0 * gi / (J->n[0] - 1), 1.0 * gj / (J->n[1] - 1),
+                     1.0 * gk / (J->n[2] - 1), px[q], py[q], pz[q]);
+        px[q] -= tl.cx;
+        py[q] -= tl.cy;
+        pz[q] -= tl.cz;
+        oidx[q] = (long long)gk + (long long)gj * J->n[2] + (long long)gi * J->n[1] * J->n[2];
+        acc[q] = 0.0;
+    }
+    int cur_t = 0, cur_pos = uniform(tbeg[0]);
+    do {
+        const int nseg =
+            fill_list<MATH>(J, src, tbeg, tl, cur_t, cur_pos, cap, s_list, s_seg, s_cnt);
+        __syncthreads();
+        eval_list<PPL, MATH>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
+        __syncthreads();
+    } while (cur_t < J->ntypes);
+#pragma unroll
+    for (int q = 0; q < PPL; ++q) {
+        if (!ok[q]) continue;
+        if (MATH == MATH_COUNT) {
+            ((uint32_t *)J->out)[oidx[q]] = (uint32_t)acc[q];
+        } else if (J->mode == OUT_F64) {
+            ((double *)J->out)[oidx[q]] = acc[q];
+        } else {
+            const double v = acc[q] < (double)FLT_MAX ? acc[q] : (double)FLT_MAX;
+            ((float *)J->out)[oidx[q]] = (float)v;
+        }
+    }
+}
+
+// Sub-list of the near list for a region of tiles: every image atom within
+// r_c + R of the region centre c, where R bounds |c_t - c| + R_t over the
+// region's tiles.  Any atom a tile's own fill would keep (d_t - R_t < r_c)
+// therefore lies in the sub-list, and the sub-list keeps the near-list order,
+// so every tile builds exactly the list it would build from the near list.
+// Returns false (and builds nothing usable) if the sub-list exceeds cap_s.
+__device__ bool build_region_list(const JobDev *__restrict__ J, double cx, double cy, double cz,
+                                  double in2, double4 *__restrict__ out, int cap_s,
+                                  int *__restrict__ rbeg, int *__restrict__ s_cnt) {
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    int n = 0;
+    for (int t = 0; t < J->ntypes; ++t) {
+        if (tid == 0) rbeg[t] = n;
+        const int b = uniform(J->type_begin[t]), e = uniform(J->type_begin[t + 1]);
+        for (int pos = b; pos < e; pos += kWG) {
+            const int idx = pos + tid;
+            bool keep = false;
+            double4 A = make_double4(0.0, 0.0, 0.0, 0.0);
+            if (idx < e) {
+                A = J->near[idx];
+                const double dx = A.x - cx, dy = A.y - cy, dz = A.z - cz;
+                keep = dx * dx + dy * dy + dz * dz < in2;
+            }
+            const uint64_t bk = __ballot(keep);
+            if (lane == 0) s_cnt[wave] = __popcll(bk);
+            __syncthreads();
+            int off = n, tot = 0;
+#pragma unroll
+            for (int w = 0; w < kWG / 64; ++w) {
+                if (w < wave) off += s_cnt[w];
+                tot += s_cnt[w];
+            }
+            __syncthreads();
+            if (n + tot > cap_s) return false;  // uniform
+            if (keep) out[off + lanes_below(bk)] = A;
+            n = uniform(n + tot);
+        }
+    }
+    if (tid == 0) rbeg[J->ntypes] = n;
+    // the sub-list is read back by every wave of the workgroup
+    __threadfence();
+    __syncthreads();
+    return true;
+}
+
 template <int PPL, int MATH>
 __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, int njobs,
                                                long long total_items,
                                                unsigned long long *__restrict__ counter,
-                                               int cap) {
+                                               int cap, double4 *__restrict__ scratch,
+                                               int scratch_cap) {
     extern __shared__ double4 s_list[];
     __shared__ int s_seg[kSegInts * kMaxSeg];
     __shared__ int s_cnt[3 * (kWG / 64)];
+    __shared__ int s_rbeg[kMaxTypes + 1];
+    __shared__ double s_red[kWG / 64];
     __shared__ long long s_item;
     const int tid = threadIdx.x;
+    double4 *__restrict__ slice =
+        scratch ? scratch + (size_t)blockIdx.x * (size_t)scratch_cap : nullptr;
 
     for (;;) {
         if (tid == 0) s_item = (long long)atomicAdd(counter, 1ull);
@@ -523,68 +635,77 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
         }
         const JobDev *__restrict__ J = jobs + jl;
         const long long local = item - J->item_begin + J->local0;
-        double lo[3], hi[3];
 
         if (J->kind == JOB_GRID) {
-            // ---- one tile of grid points, index k + j*nz + i*ny*nz ----
-            int i0[3], i1[3];
+            // ---- one region of tiles; index k + j*nz + i*ny*nz ----
             long long rem = local;
-            int tc[3];
-            tc[2] = (int)(rem % J->ntiles[2]);
-            rem /= J->ntiles[2];
-            tc[1] = (int)(rem % J->ntiles[1]);
-            tc[0] = (int)(rem / J->ntiles[1]);
+            int rg[3], t0[3], tn[3];
+            rg[2] = (int)(rem % J->nregions[2]);
+            rem /= J->nregions[2];
+            rg[1] = (int)(rem % J->nregions[1]);
+            rg[0] = (int)(rem / J->nregions[1]);
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                i0[d] = tc[d] * J->tile[d];
-                const int e = i0[d] + J->tile[d];
-                i1[d] = (e < J->n[d] ? e : J->n[d]) - 1;
-                // src/writegrid.cu:122-127
-                lo[d] = 1.0 * i0[d] / (J->n[d] - 1);
-                hi[d] = 1.0 * i1[d] / (J->n[d] - 1);
+                t0[d] = rg[d] * J->region[d];
+                const int e = t0[d] + J->region[d];
+                tn[d] = (e < J->ntiles[d] ? e : J->ntiles[d]) - t0[d];
             }
-            const Tile tl = tile_sphere(J->t, lo, hi);
-            double px[PPL], py[PPL], pz[PPL], acc[PPL];
-            long long oidx[PPL];
-            bool ok[PPL];
-            const int t1 = J->tile[1], t2 = J->tile[2];
+            const int nt = tn[0] * tn[1] * tn[2];
+            const double4 *src = J->near;
+            const int *tbeg = J->type_begin;
+            if (nt > 1 && slice && J->ntypes <= kMaxTypes) {
+                // region centre: middle of the region's point box
+                double lo[3], hi[3];
 #pragma unroll
-            for (int q = 0; q < PPL; ++q) {
-                const int p = q * kWG + tid;
-                int gi = i0[0] + p / (t1 * t2);
-                int gj = i0[1] + (p / t2) % t1;
-                int gk = i0[2] + p % t2;
-                ok[q] = gi <= i1[0] && gj <= i1[1] && gk <= i1[2];
-                if (!ok[q]) {
-                    gi = i0[0];
-                    gj = i0[1];
-                    gk = i0[2];
+                for (int d = 0; d < 3; ++d) {
+                    const int a = t0[d] * J->tile[d];
+                    const int b = (t0[d] + tn[d]) * J->tile[d];
+                    lo[d] = 1.0 * a / (J->n[d] - 1);
+                    hi[d] = 1.0 * ((b < J->n[d] ? b : J->n[d]) - 1) / (J->n[d] - 1);
                 }
-                frac_to_cart(J->t, 1.0 * gi / (J->n[0] - 1), 1.0 * gj / (J->n[1] - 1),
-                             1.0 * gk / (J->n[2] - 1), px[q], py[q], pz[q]);
-                oidx[q] = (long long)gk + (long long)gj * J->n[2] +
-                          (long long)gi * J->n[1] * J->n[2];
-                acc[q] = 0.0;
+                double cx, cy, cz;
+                frac_to_cart(J->t, 0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]),
+                             0.5 * (lo[2] + hi[2]), cx, cy, cz);
+                // R = max over tiles of |c_t - c| + R_t (one tile per thread)
+                double r = 0.0;
+                for (int ti = tid; ti < nt; ti += kWG) {
+                    const int tc[3] = {t0[0] + ti / (tn[1] * tn[2]),
+                                       t0[1] + (ti / tn[2]) % tn[1], t0[2] + ti % tn[2]};
+                    double tlo[3], thi[3];
+#pragma unroll
+                    for (int d = 0; d < 3; ++d) {
+                        const int a = tc[d] * J->tile[d];
+                        const int b = a + J->tile[d];
+                        tlo[d] = 1.0 * a / (J->n[d] - 1);
+                        thi[d] = 1.0 * ((b < J->n[d] ? b : J->n[d]) - 1) / (J->n[d] - 1);
+                    }
+                    const Tile ts = tile_sphere(J->t, tlo, thi);
+                    const double dx = ts.cx - cx, dy = ts.cy - cy, dz = ts.cz - cz;
+                    const double rt = sqrt(dx * dx + dy * dy + dz * dz) + ts.R;
+                    r = rt > r ? rt : r;
+                }
+#pragma unroll
+                for (int o = 32; o >= 1; o >>= 1) {
+                    const double v = __shfl_xor(r, o, 64);
+                    r = v > r ? v : r;
+                }
+                if ((tid & 63) == 0) s_red[tid >> 6] = r;
+                __syncthreads();
+                double R = s_red[0];
+#pragma unroll
+                for (int w = 1; w < kWG / 64; ++w) R = s_red[w] > R ? s_red[w] : R;
+                R = R * (1.0 + 1e-12) + kMargin;
+                __syncthreads();
+                const double rr = J->rc + R + kMargin;
+                if (build_region_list(J, cx, cy, cz, rr * rr, slice, scratch_cap, s_rbeg, s_cnt)) {
+                    src = slice;
+                    tbeg = s_rbeg;
+                }
             }
-            int cur_t = 0, cur_pos = uniform(J->type_begin[0]);
-            do {
-                const int nseg =
-                    fill_list<MATH>(J, tl, cur_t, cur_pos, cap, s_list, s_seg, s_cnt);
-                __syncthreads();
-                eval_list<PPL, MATH>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
-                __syncthreads();
-            } while (cur_t < J->ntypes);
-#pragma unroll
-            for (int q = 0; q < PPL; ++q) {
-                if (!ok[q]) continue;
-                if (MATH == MATH_COUNT) {
-                    ((uint32_t *)J->out)[oidx[q]] = (uint32_t)acc[q];
-                } else if (J->mode == OUT_F64) {
-                    ((double *)J->out)[oidx[q]] = acc[q];
-                } else {
-                    const double v = acc[q] < (double)FLT_MAX ? acc[q] : (double)FLT_MAX;
-                    ((float *)J->out)[oidx[q]] = (float)v;
-                }
+            for (int ti = 0; ti < nt; ++ti) {
+                const int tc[3] = {t0[0] + ti / (tn[1] * tn[2]), t0[1] + (ti / tn[2]) % tn[1],
+                                   t0[2] + ti % tn[2]};
+                grid_tile<PPL, MATH>(J, tc, src, tbeg, cap, s_list, s_seg, s_cnt);
             }
         } else {
             // ---- one Widom bin: every insertion whose point falls in it ----
@@ -592,6 +713,7 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             const int b = (int)local;
             const int bc[3] = {b / (nb1 * nb2), (b / nb2) % nb1, b % nb2};
             const int nbv[3] = {nb0, nb1, nb2};
+            double lo[3], hi[3];
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
                 lo[d] = (double)bc[d] / nbv[d];
@@ -603,8 +725,8 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             int cur_t = 0, cur_pos = uniform(J->type_begin[0]);
             bool first = true;
             do {
-                const int nseg =
-                    fill_list<MATH>(J, tl, cur_t, cur_pos, cap, s_list, s_seg, s_cnt);
+                const int nseg = fill_list<MATH>(J, J->near, J->type_begin, tl, cur_t, cur_pos,
+                                                 cap, s_list, s_seg, s_cnt);
                 __syncthreads();
                 for (uint32_t c = s0; c < s1; c += (uint32_t)(PPL * kWG)) {
                     double px[PPL], py[PPL], pz[PPL], acc[PPL];
@@ -618,6 +740,9 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                         double f[3];
                         widom_point(J->seed, J->stream, ins[q], f);
                         frac_to_cart(J->t, f[0], f[1], f[2], px[q], py[q], pz[q]);
+                        px[q] -= tl.cx;
+                        py[q] -= tl.cy;
+                        pz[q] -= tl.cz;
                         acc[q] = 0.0;
                     }
                     eval_list<PPL, MATH>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
@@ -711,9 +836,9 @@ int grid_blocks(long long n, int per_block, int cap_blocks) {
 
 template <int PPL, int MATH>
 hipError_t launch_tiles_t(const JobDev *jobs, int njobs, long long total, unsigned long long *ctr,
-                          int wgs, int cap, hipStream_t s) {
+                          int wgs, int cap, double4 *scratch, int scratch_cap, hipStream_t s) {
     hipLaunchKernelGGL((k_tiles<PPL, MATH>), dim3(wgs), dim3(kWG), tiles_lds_bytes(cap), s, jobs,
-                       njobs, total, ctr, cap);
+                       njobs, total, ctr, cap, scratch, scratch_cap);
     return hipGetLastError();
 }
 
@@ -790,11 +915,14 @@ hipError_t launch_widom_scatter(uint64_t seed, uint32_t stream, uint64_t n, cons
 size_t tiles_lds_bytes(int cap) { return (size_t)cap * sizeof(double4); }
 
 hipError_t launch_tiles(int math, int ppl, const JobDev *jobs, int njobs, long long total,
-                        unsigned long long *ctr, int wgs, int cap, hipStream_t s) {
+                        unsigned long long *ctr, int wgs, int cap, double4 *scratch,
+                        int scratch_cap, hipStream_t s) {
     if (total <= 0) return hipSuccess;
     if (cap < 2 * kWG || wgs <= 0) return hipErrorInvalidValue;
-#define ER_TILES(P, M) \
-    if (ppl == P && math == M) return launch_tiles_t<P, M>(jobs, njobs, total, ctr, wgs, cap, s);
+    if (scratch && scratch_cap < kWG) return hipErrorInvalidValue;
+#define ER_TILES(P, M)                                                                     \
+    if (ppl == P && math == M)                                                             \
+        return launch_tiles_t<P, M>(jobs, njobs, total, ctr, wgs, cap, scratch, scratch_cap, s);
     ER_TILES(1, MATH_MIXED) ER_TILES(2, MATH_MIXED)
     ER_TILES(1, MATH_FP64) ER_TILES(2, MATH_FP64)
     ER_TILES(1, MATH_COUNT) ER_TILES(2, MATH_COUNT)

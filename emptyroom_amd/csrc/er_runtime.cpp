@@ -163,6 +163,7 @@ Context::~Context() {
     pool.free(d_counter);
     pool.free(d_partials);
     pool.free(d_sums);
+    pool.free(d_scratch);
 }
 
 int context_create(int device, size_t hbm_cap, Context **out) {
@@ -190,11 +191,47 @@ int context_create(int device, size_t hbm_cap, Context **out) {
     }
     if (pool_alloc(c, 1, &c->d_counter, nullptr) != ST_OK ||
         pool_alloc(c, kReduceBlocks, &c->d_partials, nullptr) != ST_OK ||
-        pool_alloc(c, 2, &c->d_sums, nullptr) != ST_OK) {
+        pool_alloc(c, 2, &c->d_sums, nullptr) != ST_OK ||
+        scratch_ensure(c, c->stream, &c->d_scratch, &c->scratch_wgs) != ST_OK) {
         delete c;
         return ST_NOMEM;
     }
     *out = c;
+    return ST_OK;
+}
+
+int tiles_workgroups(const Context *ctx, int math, long long items, int scratch_wgs) {
+    const int occ = tiles_occupancy(math, ctx->tuning.ppl, ctx->tuning.cap);
+    long long w = (long long)ctx->n_cu * occ;
+    if (w > items) w = items;
+    if (w > scratch_wgs) w = scratch_wgs;
+    return (int)std::max<long long>(w, 1);
+}
+
+int scratch_ensure(Context *ctx, hipStream_t s, double4 **buf, int *wgs) {
+    int occ = 1;
+    for (int m : {(int)MATH_MIXED, (int)MATH_FP64, (int)MATH_COUNT})
+        occ = std::max(occ, tiles_occupancy(m, ctx->tuning.ppl, ctx->tuning.cap));
+    int need = ctx->n_cu * occ;
+    if (*buf && *wgs >= need) return ST_OK;
+    // Under a tight HBM cap take fewer slices: launches then use fewer
+    // workgroups, which changes no result.
+    void *p = nullptr;
+    for (; need >= 1 && need > *wgs; need /= 2) {
+        p = ctx->pool.alloc((size_t)need * kRegionRecords * sizeof(double4));
+        if (p) break;
+    }
+    if (!p) {
+        if (*buf) return ST_OK;  // keep the smaller scratch
+        set_error("no device memory for the region scratch");
+        return ST_NOMEM;
+    }
+    if (*buf) {
+        HIP_TRY(hipStreamSynchronize(s));
+        ctx->pool.free(*buf);
+    }
+    *buf = (double4 *)p;
+    *wgs = need;
     return ST_OK;
 }
 
@@ -263,8 +300,6 @@ int framework_upload(Context *ctx, const Particle *atoms, int n, const double t[
             tp.sig2 = atoms[i].sig2;
             tp.eps4 = 4.0 * atoms[i].eps;
             tp.rclose = std::sqrt(std::max(atoms[i].sig2, 0.0) / ctx->tuning.s2_switch);
-            tp.sig2f = (float)tp.sig2;
-            tp.eps4f = (float)tp.eps4;
             fw->types.push_back(tp);
             type_of[i] = id;
         } else {
@@ -368,6 +403,57 @@ void choose_widom_bins(const FrameworkDev *fw, uint64_t n_ins, const Tuning &tu,
     for (int d = 0; d < 3; d++) nb[d] = std::max(1, std::min(1 << 10, (int)std::lround(w[d] * k)));
 }
 
+// Tiles per region per axis: about region_len Angstrom along each lattice
+// vector, shrunk (largest extent first) until the expected sub-list -- the
+// framework's atom density times the sphere of radius r_c + region radius --
+// fills at most 3/4 of a workgroup's scratch and the grid has at least
+// kMinGridItems regions (a small grid alone must still fill the device).
+// Regions of one tile skip the sub-list.  Depends only on the framework, the
+// grid and the tuning.
+void choose_grid_region(const FrameworkDev *fw, const int n[3], const int tile[3],
+                        const Tuning &tu, int region[3]) {
+    double edge[3];
+    for (int d = 0; d < 3; d++) {
+        const double a = std::sqrt(fw->t[d] * fw->t[d] + fw->t[3 + d] * fw->t[3 + d] +
+                                   fw->t[6 + d] * fw->t[6 + d]);
+        edge[d] = a * tile[d] / (n[d] - 1);
+        region[d] = 1;
+        if (tu.region_len > 0.0 && edge[d] > 0.0)
+            region[d] = std::max(1, (int)std::lround(tu.region_len / edge[d]));
+        region[d] = std::min(region[d], 4096);
+    }
+    const double pi = 3.14159265358979323846;
+    const double dens = fw->n_atoms / std::max(fw->volume, 1e-9);
+    for (;;) {
+        // radius of the region box: largest centre-to-corner distance
+        double r2 = 0.0;
+        for (int c = 0; c < 8; c++) {
+            double f[3];
+            for (int d = 0; d < 3; d++)
+                f[d] = ((c >> d) & 1 ? 0.5 : -0.5) * region[d] * tile[d] / (n[d] - 1);
+            double x = 0, y = 0, z = 0;
+            x = fw->t[0] * f[0] + fw->t[1] * f[1] + fw->t[2] * f[2];
+            y = fw->t[3] * f[0] + fw->t[4] * f[1] + fw->t[5] * f[2];
+            z = fw->t[6] * f[0] + fw->t[7] * f[1] + fw->t[8] * f[2];
+            r2 = std::max(r2, x * x + y * y + z * z);
+        }
+        const double rr = fw->rc + std::sqrt(r2) + 0.01;
+        const double expect = dens * (4.0 / 3.0) * pi * rr * rr * rr;
+        // keep enough work items for a launch of this job alone to fill the device
+        long long items = 1;
+        for (int d = 0; d < 3; d++) {
+            const long long nt = (n[d] + tile[d] - 1) / tile[d];
+            items *= (nt + region[d] - 1) / region[d];
+        }
+        if (expect <= 0.75 * kRegionRecords && items >= kMinGridItems) break;
+        int big = 0;
+        for (int d = 1; d < 3; d++)
+            if (region[d] * edge[d] > region[big] * edge[big]) big = d;
+        if (region[big] == 1) break;
+        region[big] = (region[big] + 1) / 2;
+    }
+}
+
 JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void *d_out,
                      const Tuning &tu, int math) {
     JobDev j;
@@ -382,14 +468,17 @@ JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void
     j.ntypes = fw->ntypes;
     j.kind = JOB_GRID;
     j.mode = math == MATH_COUNT ? OUT_COUNT : (precision == 1 ? OUT_F64 : OUT_F32);
-    int tile[3];
+    int tile[3], region[3];
     choose_grid_tile(fw, n, tu, tile);
+    choose_grid_region(fw, n, tile, tu, region);
     long long items = 1;
     for (int d = 0; d < 3; d++) {
         j.n[d] = n[d];
         j.tile[d] = tile[d];
         j.ntiles[d] = (n[d] + tile[d] - 1) / tile[d];
-        items *= j.ntiles[d];
+        j.region[d] = std::min(region[d], j.ntiles[d]);
+        j.nregions[d] = (j.ntiles[d] + j.region[d] - 1) / j.region[d];
+        items *= j.nregions[d];
     }
     j.out = d_out;
     j.n_items = items;
@@ -455,10 +544,7 @@ int check_tuning(const Tuning &tu) {
 }
 
 int persistent_wgs(Context *ctx, int math, long long items) {
-    const int occ = tiles_occupancy(math, ctx->tuning.ppl, ctx->tuning.cap);
-    long long w = (long long)ctx->n_cu * occ;
-    if (w > items) w = items;
-    return (int)std::max<long long>(w, 1);
+    return tiles_workgroups(ctx, math, items, ctx->scratch_wgs);
 }
 
 // Launch one job array on the context stream with the context's counter.
@@ -467,7 +553,8 @@ int run_jobs(Context *ctx, int math, JobDev *d_jobs, int njobs, long long total,
     hipStream_t s = ctx->stream;
     HIP_TRY(launch_fill_u64(d_counter, 0ull, 1, s));
     HIP_TRY(launch_tiles(math, ctx->tuning.ppl, d_jobs, njobs, total, d_counter,
-                         persistent_wgs(ctx, math, total), ctx->tuning.cap, s));
+                         persistent_wgs(ctx, math, total), ctx->tuning.cap, ctx->d_scratch,
+                         kRegionRecords, s));
     return ST_OK;
 }
 
@@ -693,7 +780,8 @@ double job_cost_per_item(const FrameworkDev *fw, const JobDev &j, const Tuning &
     const double pi = 3.14159265358979323846;
     const double dens = fw->n_atoms / std::max(fw->volume, 1e-9);
     double pts;
-    if (j.kind == JOB_GRID) pts = (double)j.tile[0] * j.tile[1] * j.tile[2];
+    if (j.kind == JOB_GRID)
+        pts = (double)j.tile[0] * j.tile[1] * j.tile[2] * j.region[0] * j.region[1] * j.region[2];
     else pts = (double)tu.widom_per_bin;
     return pts * dens * (4.0 / 3.0) * pi * fw->rc2 * fw->rc;
 }
@@ -720,7 +808,8 @@ int Batch::record(hipStream_t s) {
     if (items_mixed > 0) {
         if (ev_on) HIP_TRY(hipEventRecord(ev[0], s));
         HIP_TRY(launch_tiles(MATH_MIXED, ctx->tuning.ppl, d_jobs, n_mixed, items_mixed, d_counter,
-                             persistent_wgs(ctx, MATH_MIXED, items_mixed), ctx->tuning.cap, s));
+                             persistent_wgs(ctx, MATH_MIXED, items_mixed), ctx->tuning.cap,
+                             ctx->d_scratch, kRegionRecords, s));
         if (ev_on) HIP_TRY(hipEventRecord(ev[1], s));
         timed_launches |= 1;
     }
@@ -728,7 +817,8 @@ int Batch::record(hipStream_t s) {
         if (ev_on) HIP_TRY(hipEventRecord(ev[2], s));
         HIP_TRY(launch_tiles(MATH_FP64, ctx->tuning.ppl, d_jobs + n_mixed,
                              (int)jobs.size() - n_mixed, items_fp64, d_counter + 1,
-                             persistent_wgs(ctx, MATH_FP64, items_fp64), ctx->tuning.cap, s));
+                             persistent_wgs(ctx, MATH_FP64, items_fp64), ctx->tuning.cap,
+                             ctx->d_scratch, kRegionRecords, s));
         if (ev_on) HIP_TRY(hipEventRecord(ev[3], s));
         timed_launches |= 2;
     }

@@ -46,8 +46,9 @@ struct Tuning {
     int ppl = 1;                 // points per lane (1 or 2)
     int cap = 1024;              // LDS list capacity (records of 32 B)
     int tile[3] = {8, 8, 4};     // grid tile in points, product = 256 * ppl
-    int widom_per_bin = 256;     // target insertions per Widom bin
-    double s2_switch = 0.25;     // (sigma/r)^2 above which a pair takes the FP64 path
+    int widom_per_bin = 512;     // target insertions per Widom bin
+    double s2_switch = 0.444;    // (sigma/r)^2 above which a pair takes the FP64 path (r < 1.5 sigma)
+    double region_len = 4.0;     // edge of a region of grid tiles in Angstrom (0: no regions)
 };
 
 class Context;
@@ -83,10 +84,19 @@ class Context {
     unsigned long long *d_counter = nullptr;  // work counter of plain launches
     double2 *d_partials = nullptr;
     double *d_sums = nullptr;
+    double4 *d_scratch = nullptr;  // region sub-lists, scratch_wgs * kRegionRecords records
+    int scratch_wgs = 0;
     ~Context();
 };
 
 int context_create(int device, size_t hbm_cap, Context **out);
+// Persistent workgroups a launch of the tile kernel may use for `items` work
+// items, given the tuning's occupancy and a scratch of `scratch_wgs` slices.
+int tiles_workgroups(const Context *ctx, int math, long long items, int scratch_wgs);
+// Region scratch for every workgroup the current tuning can keep resident
+// (n_cu * occupancy); grows *buf if needed.  Called at context creation and
+// after a tuning change; never during graph capture.
+int scratch_ensure(Context *ctx, hipStream_t s, double4 **buf, int *wgs);
 
 struct FrameworkDev {
     Context *ctx = nullptr;
@@ -141,6 +151,8 @@ struct WidomBuffers {
 };
 void choose_widom_bins(const FrameworkDev *fw, uint64_t n_ins, const Tuning &tu, int nb[3]);
 void choose_grid_tile(const FrameworkDev *fw, const int n[3], const Tuning &tu, int tile[3]);
+void choose_grid_region(const FrameworkDev *fw, const int n[3], const int tile[3],
+                        const Tuning &tu, int region[3]);
 JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void *d_out,
                      const Tuning &tu, int math);
 JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,

@@ -44,6 +44,8 @@ struct Server::Launch {
     JobDev *h_jobs = nullptr;     // pinned
     JobDev *d_jobs = nullptr;
     unsigned long long *d_counter = nullptr;
+    double4 *d_scratch = nullptr;  // region sub-lists of this launch slot
+    int scratch_wgs = 0;
     std::vector<std::pair<Req *, long long>> parts;
     bool busy = false;
 };
@@ -59,9 +61,12 @@ Server::Server(Context *ctx, const ServerConfig &cfg) : ctx_(ctx), cfg_(cfg) {
         (void)hipHostMalloc((void **)&L->h_jobs, sizeof(JobDev) * kMaxLaunchJobs);
         L->d_jobs = (JobDev *)ctx_->pool.alloc(sizeof(JobDev) * kMaxLaunchJobs);
         L->d_counter = (unsigned long long *)ctx_->pool.alloc(sizeof(unsigned long long));
+        if (!L->stream || !L->done || !L->h_jobs || !L->d_jobs || !L->d_counter ||
+            scratch_ensure(ctx_, L->stream, &L->d_scratch, &L->scratch_wgs) != ST_OK)
+            status_ = ST_NOMEM;
         launches_.push_back(std::move(L));
     }
-    worker_ = std::thread([this] { run(); });
+    if (status_ == ST_OK) worker_ = std::thread([this] { run(); });
 }
 
 Server::~Server() {
@@ -78,6 +83,7 @@ Server::~Server() {
         if (L->h_jobs) (void)hipHostFree(L->h_jobs);
         ctx_->pool.free(L->d_jobs);
         ctx_->pool.free(L->d_counter);
+        ctx_->pool.free(L->d_scratch);
     }
     for (auto &kv : all_) {
         Req *r = kv.second.get();
@@ -280,7 +286,11 @@ void Server::run() {
                     long long &next = kind == 0 ? r->grid_next : r->widom_next;
                     const long long left = src.n_items - next;
                     if (left <= 0) continue;
-                    const long long take = std::min(left, std::min(share, budget));
+                    // a grid item is a region of tiles; a Widom item is one bin
+                    const long long w =
+                        kind == 0 ? (long long)src.region[0] * src.region[1] * src.region[2] : 1;
+                    const long long take =
+                        std::min(left, std::max<long long>(1, std::min(share, budget) / w));
                     JobDev j = src;
                     j.local0 = next;
                     j.n_items = take;
@@ -289,7 +299,7 @@ void Server::run() {
                     L->parts.push_back({r, take});
                     next += take;
                     total += take;
-                    budget -= take;
+                    budget -= take * w;
                 }
             }
             if (nj == 0) break;
@@ -327,11 +337,9 @@ void Server::run() {
             (void)hipMemcpyAsync(L->d_jobs, L->h_jobs, sizeof(JobDev) * nj, hipMemcpyHostToDevice,
                                  L->stream);
             (void)launch_fill_u64(L->d_counter, 0ull, 1, L->stream);
-            const int occ = tiles_occupancy(math, ctx_->tuning.ppl, ctx_->tuning.cap);
-            const int wgs = (int)std::max<long long>(
-                1, std::min<long long>(total, (long long)ctx_->n_cu * occ));
+            const int wgs = tiles_workgroups(ctx_, math, total, L->scratch_wgs);
             (void)launch_tiles(math, ctx_->tuning.ppl, L->d_jobs, nj, total, L->d_counter, wgs,
-                               ctx_->tuning.cap, L->stream);
+                               ctx_->tuning.cap, L->d_scratch, kRegionRecords, L->stream);
             (void)hipEventRecord(L->done, L->stream);
             L->busy = true;
             n_launches_++;

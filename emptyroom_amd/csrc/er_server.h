@@ -30,7 +30,7 @@ namespace er {
 
 struct ServerConfig {
     int streams = 3;                  // launches in flight
-    long long items_per_launch = 4096;  // work items per launch (0 = all pending)
+    long long items_per_launch = 32768;  // grid tiles + Widom bins per launch (0 = all pending)
     int max_queue = 4096;             // waiting requests beyond which submit rejects
 };
 
@@ -61,6 +61,8 @@ class Server {
   public:
     Server(Context *ctx, const ServerConfig &cfg);
     ~Server();
+    // ST_OK, or ST_NOMEM if the launch slots could not get their buffers.
+    int status() const { return status_; }
     // Returns ST_OK and a ticket, or ST_CAPACITY if the queue is full, or
     // ST_NOMEM if the request alone exceeds the HBM cap (both count as rejections).
     int submit(const RequestSpec &spec, uint64_t *ticket);
@@ -94,6 +96,7 @@ class Server {
     std::atomic<bool> stop_{false};
     long long n_submitted_ = 0, n_rejected_ = 0, n_completed_ = 0, n_launches_ = 0;
     std::thread worker_;
+    int status_ = ST_OK;
 };
 
 }  // namespace er

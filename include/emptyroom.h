@@ -79,6 +79,9 @@ typedef struct er_tuning {
     int tile[3];           /* grid tile in points, product = 256 * points_per_lane */
     int widom_per_bin;     /* target insertions per Widom bin */
     double s2_switch;      /* (sigma/r)^2 above which an FP32-grid pair uses FP64 */
+    double region_len;     /* edge in Angstrom of the tile regions that share one
+                              candidate sub-list (0 = every tile scans the whole
+                              near list) */
 } er_tuning;
 
 const char *er_last_error(void);
