@@ -243,6 +243,46 @@ def bench_pockets(args, api, synth):
     }
 
 
+def bench_copy():
+    """Plain 1 GiB device-to-device copy (3 times): the known-size reference
+    that scales the FETCH_SIZE / WRITE_SIZE counters of the other kernels."""
+    import torch
+    n = 256 << 20
+    x = torch.ones(n, dtype=torch.float32, device="cuda")
+    y = torch.empty_like(x)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(3):
+        y.copy_(x)
+    torch.cuda.synchronize()
+    dt = (time.perf_counter() - t0) / 3
+    return {"metric": "device copy bandwidth (counter calibration)", "value": 2 * 4 * n / dt / 1e9,
+            "unit": "GB/s", "n_gpus": 1, "steps": 3, "warmup": 0, "ms_per_step": dt * 1e3,
+            "higher_is_better": True, "config": {"workload": "copy: 1 GiB fp32, 3 times"}}
+
+
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "k_tiles_traffic_config3_256fw.txt")
+
+
+def measured_traffic(tuning, n_frameworks):
+    """HBM bytes per launch of the batched k_tiles dispatch from the committed
+    rocprofv3 FETCH_SIZE / WRITE_SIZE runs (scaled by the copy calibration),
+    if they were taken on this workload and tuning; else None."""
+    try:
+        kv = {}
+        with open(TRAFFIC_FILE) as fh:
+            for line in fh:
+                if ":" in line and not line.startswith("#"):
+                    k, v = line.split(":", 1)
+                    kv[k.strip()] = v.strip()
+        if int(kv["frameworks"]) != n_frameworks or json.loads(kv["tuning"]) != tuning:
+            return None
+        return (float(kv["fetch_kib"]) * float(kv["fetch_scale"]) +
+                float(kv["write_kib"]) * float(kv["write_scale"])) * 1024.0
+    except (OSError, KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -269,6 +309,9 @@ def main():
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
         sys.exit(2)
 
+    if args.workload == "copy":
+        print(json.dumps(bench_copy()))
+        return
     from emptyroom_amd import api, synth
     if args.workload == "config4":
         if args.frameworks == 256:
@@ -401,7 +444,9 @@ def main():
         "grid_max_abs_err_K": grid_err,
         "roofline": {"bound": "mfma", "engine": "VALU FP32 (no MFMA: pairwise LJ)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
-                     "frac": achieved / PEAK_FP32_TFLOPS, "traffic": None,
+                     "frac": achieved / PEAK_FP32_TFLOPS,
+                     "traffic": measured_traffic(tuning, len(fws))
+                     if args.workload == "config3" else None,
                      "kernel": f"k_tiles<{tuning['ppl']},MIXED>", "kernel_ms": kern_s * 1e3,
                      "flop_per_pair": FLOP_PER_PAIR},
         "tuning": tuning,

@@ -103,11 +103,13 @@ EXPORTS = [
     "er_grid_info", "er_grid_release", "er_widom", "er_count_incut_grid",
     "er_count_incut_widom", "er_widom_point", "er_henry_result", "er_batch_create", "er_batch_run",
     "er_batch_wait", "er_batch_result", "er_batch_release", "er_batch_set_timing",
-    "er_batch_kernel_ms", "er_grid_upload", "er_pockets_label", "er_widom_grid",
+    "er_batch_kernel_ms", "er_grid_upload", "er_pockets_label", "er_pockets_label_batch",
+    "er_widom_grid",
     "er_server_create", "er_server_submit", "er_server_poll", "er_server_fetch_grid",
     "er_server_release", "er_server_stats", "er_server_destroy", "er_host_setup",
     "er_host_run_info", "er_host_run_particles", "er_host_run_label",
     "er_host_run_write_settings", "er_host_run_release", "er_write_grid_file",
+    "er_replication_factors", "er_write_uc_file",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -155,6 +157,9 @@ _SIGS = {
     "er_grid_upload": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int), ctypes.POINTER(_P)]),
     "er_pockets_label": (ctypes.c_int, [_P, _P, ctypes.c_double, ctypes.c_int,
                                         ctypes.POINTER(ctypes.c_int), _P]),
+    "er_pockets_label_batch": (ctypes.c_int, [_P, ctypes.POINTER(_P), ctypes.c_int,
+                                              ctypes.c_double, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int)]),
     "er_widom_grid": (ctypes.c_int, [_P, _P, ctypes.c_uint64, ctypes.c_double, ctypes.c_uint64,
                                      ctypes.POINTER(WidomResult), _P]),
     "er_server_create": (ctypes.c_int, [_P, ctypes.POINTER(ServerConfig), ctypes.POINTER(_P)]),
@@ -176,6 +181,10 @@ _SIGS = {
     "er_write_grid_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P, _P,
                                           ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "er_replication_factors": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int,
+                                              ctypes.POINTER(ctypes.c_int)]),
+    "er_write_uc_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double,
+                                        ctypes.c_int, ctypes.POINTER(ctypes.c_int)]),
 }
 
 
@@ -429,6 +438,16 @@ def pockets_label(ctx: Context, grid: Grid, temperature: float, block: bool = Tr
                                   ctypes.byref(npk),
                                   seg.ctypes.data_as(ctypes.c_void_p) if segments else None))
     return npk.value, seg
+
+
+def pockets_label_batch(ctx: Context, grids: Sequence[Grid], temperature: float,
+                        block: bool = True):
+    """Pocket counts of many grids labelled (and blocked) together."""
+    arr = (_P * max(len(grids), 1))(*[g.h for g in grids])
+    out = (ctypes.c_int * max(len(grids), 1))()
+    _check(lib().er_pockets_label_batch(ctx.h, arr, len(grids), float(temperature), int(block),
+                                        out))
+    return [out[i] for i in range(len(grids))]
 
 
 def widom_grid(ctx: Context, grid: Grid, n: int, temperature: float, seed: int,

@@ -308,6 +308,18 @@ int er_pockets_label(er_context *ctx, er_grid *grid, double temperature, int blo
     });
 }
 
+int er_pockets_label_batch(er_context *ctx, er_grid *const *grids, int n_grids,
+                           double temperature, int block_in_place, int *n_pockets) {
+    if (!ctx || (n_grids > 0 && (!grids || !n_pockets)) || n_grids < 0)
+        return arg_error("er_pockets_label_batch: bad arguments");
+    return guarded([&]() -> int {
+        std::vector<er::GridDev *> g(n_grids);
+        for (int i = 0; i < n_grids; i++) g[i] = G(grids[i]);
+        return er::pockets_label_many(C(ctx), g.data(), n_grids, temperature,
+                                      block_in_place != 0, n_pockets, nullptr);
+    });
+}
+
 int er_widom_grid(er_context *ctx, er_grid *grid, uint64_t n_insertions, double temperature,
                   uint64_t seed, er_widom_result *out, double *energies_out) {
     if (!ctx || !grid || !out) return arg_error("er_widom_grid: null argument");
@@ -544,6 +556,35 @@ int er_write_grid_file(const char *path, const char *format, const double *e64, 
     std::fclose(f);
     if (st != ST_OK) er::set_error("er_write_grid_file: write failed");
     return st;
+}
+
+int er_replication_factors(const double t_matrix[9], double cutoff, int twice, int rep[3]) {
+    if (!t_matrix || !rep) return arg_error("er_replication_factors: null argument");
+    int st = er::host::replication_factors(t_matrix, cutoff, twice != 0, rep);
+    if (st != ST_OK) er::set_error("er_replication_factors: degenerate cell or negative cutoff");
+    return st;
+}
+
+int er_write_uc_file(const char *root, const char *structure, double cutoff, int twice,
+                     int rep[3]) {
+    if (!root || !structure) return arg_error("er_write_uc_file: null argument");
+    return guarded([&]() -> int {
+        std::string msg;
+        double t[9];
+        int r[3];
+        int st = er::host::read_cssr_cell(root, structure, t, &msg);
+        if (st == ST_OK) {
+            st = er::host::replication_factors(t, cutoff, twice != 0, r);
+            if (st != ST_OK) msg = "degenerate cell or negative cutoff";
+        }
+        if (st == ST_OK) st = er::host::write_uc_file(root, structure, twice != 0, r, &msg);
+        if (st != ST_OK) {
+            er::set_error(msg);
+            return st;
+        }
+        if (rep) std::memcpy(rep, r, sizeof(r));
+        return ST_OK;
+    });
 }
 
 }  // extern "C"

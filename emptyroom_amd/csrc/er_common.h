@@ -105,10 +105,26 @@ struct JobDev {
     long long local0;      // first tile / bin of the job covered by this launch
 };
 
+// One Widom binning job: insertion points of (seed, stream) sorted into a
+// regular grid of fractional bins.  Plain-old-data; kernels take an array.
+struct BinJob {
+    uint64_t seed;
+    uint64_t n;            // insertions
+    uint32_t stream;
+    int nb[3];             // bins per fractional axis
+    uint32_t *count;       // nbins + 1 counts (zeroed by k_bin_zero)
+    uint32_t *start;       // nbins + 1 exclusive prefix sums
+    uint32_t *cursor;      // nbins fill cursors
+    uint32_t *sorted;      // n insertion indices, grouped by bin
+    long long block0;      // first binning block of this job in a launch
+};
+
 // Kernel launch geometry.
 constexpr int kWG = 256;      // threads per workgroup (4 wave64)
 constexpr int kMaxSeg = 64;   // list segments per fill pass
 constexpr int kMaxTypes = 64; // pair types a region sub-list can index
+constexpr int kLdsBins = 8192; // Widom bins a binning block histograms in LDS
+constexpr int kBinChunk = 16384;  // insertions per binning block
 constexpr int kRegionRecords = 8192;  // records of one workgroup's region sub-list (256 KiB)
 constexpr long long kMinGridItems = 2048;  // regions per grid before regions shrink
 

@@ -366,6 +366,92 @@ void flat_t(const Framework &fw, double t[9]) {
 
 namespace {
 
+void cross3(const double u[3], const double v[3], double w[3]) {
+    w[0] = u[1] * v[2] - u[2] * v[1];
+    w[1] = u[2] * v[0] - u[0] * v[2];
+    w[2] = u[0] * v[1] - u[1] * v[0];
+}
+
+double dot3(const double u[3], const double v[3]) { return u[0] * v[0] + u[1] * v[1] + u[2] * v[2]; }
+
+}  // namespace
+
+int replication_factors(const double t[9], double cutoff, bool twice, int rep[3]) {
+    const double rc = twice ? 2.0 * cutoff : cutoff;
+    double v[3][3];  // cell vectors = columns of t
+    for (int k = 0; k < 3; k++)
+        for (int i = 0; i < 3; i++) v[k][i] = t[3 * i + k];
+    // normals of the faces spanned by the other two (original) vectors
+    double nrm[3][3];
+    cross3(v[1], v[2], nrm[0]);
+    cross3(v[0], v[2], nrm[1]);
+    cross3(v[0], v[1], nrm[2]);
+    for (int k = 0; k < 3; k++) {
+        rep[k] = 1;
+        if (!(dot3(nrm[k], nrm[k]) > 0.0) || !(rc >= 0.0)) return ST_ARG;
+    }
+    for (int k = 0; k < 3; k++) {
+        const double len = std::sqrt(dot3(nrm[k], nrm[k]));
+        for (;;) {
+            double c0[3];
+            for (int i = 0; i < 3; i++) c0[i] = 0.5 * (v[0][i] + v[1][i] + v[2][i]);
+            if (std::fabs(dot3(nrm[k], c0)) / len >= rc) break;
+            if (rep[k] >= 64) return ST_ARG;
+            rep[k]++;
+            for (int i = 0; i < 3; i++) v[k][i] *= 2.0;
+        }
+    }
+    return ST_OK;
+}
+
+int read_cssr_cell(const std::string &root, const std::string &name, double t[9],
+                   std::string *msg) {
+    std::ifstream cssr(join(root, "data/structures/" + name + ".cssr").c_str());
+    if (cssr.fail()) {
+        *msg = fmt("Crystal structure file %s not present in data/structures/",
+                   (name + ".cssr").c_str());
+        return ST_IO;
+    }
+    double a = 0, b = 0, c = 0, al = 0, be = 0, ga = 0;
+    std::string line;
+    std::getline(cssr, line);
+    std::istringstream input(line);
+    input >> a >> b >> c;
+    std::getline(cssr, line);
+    input.str(line);
+    input.clear();
+    input >> al >> be >> ga;
+    if (input.fail()) {
+        *msg = fmt("CSSR file %s has no cell", (name + ".cssr").c_str());
+        return ST_PARSE;
+    }
+    al *= M_PI / 180.0;
+    be *= M_PI / 180.0;
+    ga *= M_PI / 180.0;
+    const double ca = std::cos(al), cb = std::cos(be), cg = std::cos(ga), sg = std::sin(ga);
+    const double vu = std::sqrt(1.0 - ca * ca - cb * cb - cg * cg + 2.0 * ca * cb * cg);
+    const double m[9] = {a, b * cg, c * cb, 0.0, b * sg, c * (ca - cb * cg) / sg,
+                         0.0, 0.0, c * vu / sg};
+    std::memcpy(t, m, sizeof(m));
+    return ST_OK;
+}
+
+int write_uc_file(const std::string &root, const std::string &name, bool twice,
+                  const int rep[3], std::string *msg) {
+    const std::string path =
+        join(root, "data/uc_replications/" + name + (twice ? "_twice" : "_once") + ".uc");
+    FILE *f = fopen(path.c_str(), "w");
+    if (!f) {
+        *msg = fmt("could not write %s", path.c_str());
+        return ST_IO;
+    }
+    fprintf(f, "%d %d %d", rep[0], rep[1], rep[2]);
+    fclose(f);
+    return ST_OK;
+}
+
+namespace {
+
 // Shared leading block of both .out headers
 // (src/write_settings_to_outputfile.h:15-90 and :107-176).
 void write_common_settings(FILE *f, const std::string &adsorbate, double mw, double eps_g,

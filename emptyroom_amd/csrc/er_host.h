@@ -125,5 +125,21 @@ int setup_henry_run(const std::string &root, const std::string &structure,
 // Flattened fractional->Cartesian matrix.
 void flat_t(const Framework &fw, double t[9]);
 
+// Replication factors (julia_codes/framework.jl:162-211): starting from 1,
+// an axis's factor grows while the distance from the centre of the current
+// supercell to the face plane spanned by the other two cell vectors is below
+// the cutoff ("once") or twice the cutoff ("twice"); each step doubles that
+// cell vector (not (r+1)/r), so r = 1 + ceil(log2(2 r_c / w)) for a
+// perpendicular width w < 2 r_c.  Returns ST_ARG for a degenerate cell.
+int replication_factors(const double t[9], double cutoff, bool twice, int rep[3]);
+// Cell of data/structures/<name>.cssr (first two lines; no masses file is
+// needed, as in julia_codes/framework.jl:58-82), as a flattened matrix.
+int read_cssr_cell(const std::string &root, const std::string &name, double t[9],
+                   std::string *msg);
+// data/uc_replications/<name>_<once|twice>.uc holding "%d %d %d"
+// (julia_codes/framework.jl:213-227).
+int write_uc_file(const std::string &root, const std::string &name, bool twice,
+                  const int rep[3], std::string *msg);
+
 }  // namespace host
 }  // namespace er

@@ -42,17 +42,17 @@ hipError_t launch_near_offsets(const NearPlan &p, const uint32_t *block_count,
 hipError_t launch_near_write(const NearPlan &p, const uint32_t *block_off, double4 *near,
                              hipStream_t s);
 
-// --- Widom binning (per job) -------------------------------------------------
-hipError_t launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s);
+// --- fills ------------------------------------------------------------------
 hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, long long n,
                            hipStream_t s);
-hipError_t launch_widom_count(uint64_t seed, uint32_t stream, uint64_t n, const int nbins[3],
-                              uint32_t *count, hipStream_t s);
-// exclusive scan of n counts into start[0..n] (start[n] = total), cursor = start
-hipError_t launch_scan_u32(const uint32_t *count, long long n, uint32_t *start,
-                           uint32_t *cursor, hipStream_t s);
-hipError_t launch_widom_scatter(uint64_t seed, uint32_t stream, uint64_t n, const int nbins[3],
-                                uint32_t *cursor, uint32_t *sorted, hipStream_t s);
+
+// --- Widom binning (batched) --------------------------------------------------
+// jobs: device array of njobs BinJob whose block0 fields number the blocks of
+// `chunk` insertions each; total_blocks = sum over jobs of ceil(n / chunk).
+// Order: zero, count, scan, scatter (all on one stream).
+hipError_t launch_bin_all(const BinJob *jobs, int njobs, long long total_blocks, int chunk,
+                          long long max_bins, hipStream_t s);
+int bin_blocks(uint64_t n, int chunk);
 
 // --- tiled energy kernel ------------------------------------------------------
 // Runs the work items [0, total_items) of the jobs (device array) with a

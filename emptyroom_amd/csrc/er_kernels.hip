@@ -152,11 +152,6 @@ __global__ __launch_bounds__(kWG) void k_near_offsets(NearPlan p, const uint32_t
     }
 }
 
-__global__ __launch_bounds__(kWG) void k_scan_u32(const uint32_t *count, long long n,
-                                                  uint32_t *start, uint32_t *cursor) {
-    block_exclusive_scan(count, n, start, cursor);
-}
-
 // ---------------------------------------------------------------------------
 // Widom binning
 // ---------------------------------------------------------------------------
@@ -170,34 +165,102 @@ __device__ __forceinline__ uint32_t widom_bin(const double f[3], const int nb[3]
     return (uint32_t)((b[0] * nb[1] + b[1]) * nb[2] + b[2]);
 }
 
-__global__ __launch_bounds__(kWG) void k_widom_count(uint64_t seed, uint32_t stream, uint64_t n,
-                                                     int nb0, int nb1, int nb2, uint32_t *count) {
-    const int nb[3] = {nb0, nb1, nb2};
-    for (uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * kWG) {
-        double f[3];
-        widom_point(seed, stream, i, f);
-        atomicAdd(&count[widom_bin(f, nb)], 1u);
+// Batched binning.  Each block takes `chunk` consecutive insertion indices of
+// one job, histograms their bins in LDS and touches global memory once per
+// bin it saw: the count pass adds its counts, the scatter pass reserves a
+// range of the bin's slots and hands them out through LDS atomics.  The order
+// inside a bin is therefore arbitrary; no result depends on it (every
+// insertion of a bin sees the same candidate list, and sums run over
+// insertion indices).  Jobs with more than kLdsBins bins use one global
+// atomic per insertion.
+__device__ __forceinline__ int bin_job_of(const BinJob *__restrict__ jobs, int njobs,
+                                          long long blk) {
+    int lo = 0, hi = njobs - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (jobs[mid].block0 <= blk) lo = mid;
+        else hi = mid - 1;
     }
+    return lo;
 }
 
-__global__ __launch_bounds__(kWG) void k_widom_scatter(uint64_t seed, uint32_t stream, uint64_t n,
-                                                       int nb0, int nb1, int nb2,
-                                                       uint32_t *cursor, uint32_t *sorted) {
-    const int nb[3] = {nb0, nb1, nb2};
-    for (uint64_t i = (uint64_t)blockIdx.x * kWG + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * kWG) {
-        double f[3];
-        widom_point(seed, stream, i, f);
-        uint32_t pos = atomicAdd(&cursor[widom_bin(f, nb)], 1u);
-        sorted[pos] = (uint32_t)i;
-    }
+__global__ __launch_bounds__(kWG) void k_bin_zero(const BinJob *__restrict__ jobs) {
+    const BinJob &J = jobs[blockIdx.x];
+    const long long nbt = (long long)J.nb[0] * J.nb[1] * J.nb[2];
+    for (long long b = threadIdx.x; b <= nbt; b += kWG) J.count[b] = 0u;
 }
 
-__global__ void k_fill_u32(uint32_t *p, uint32_t v, long long n) {
-    for (long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (long long)gridDim.x * blockDim.x)
-        p[i] = v;
+__global__ __launch_bounds__(kWG) void k_bin_count(const BinJob *__restrict__ jobs, int njobs,
+                                                   int chunk) {
+    __shared__ uint32_t s_hist[kLdsBins];
+    const int j = bin_job_of(jobs, njobs, blockIdx.x);
+    const BinJob &J = jobs[j];
+    const uint64_t i0 = (uint64_t)(blockIdx.x - J.block0) * (uint64_t)chunk;
+    const uint64_t i1 = i0 + (uint64_t)chunk < J.n ? i0 + (uint64_t)chunk : J.n;
+    const int nbt = J.nb[0] * J.nb[1] * J.nb[2];
+    if (nbt > kLdsBins) {
+        for (uint64_t i = i0 + threadIdx.x; i < i1; i += kWG) {
+            double f[3];
+            widom_point(J.seed, J.stream, i, f);
+            atomicAdd(&J.count[widom_bin(f, J.nb)], 1u);
+        }
+        return;
+    }
+    for (int b = threadIdx.x; b < nbt; b += kWG) s_hist[b] = 0u;
+    __syncthreads();
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kWG) {
+        double f[3];
+        widom_point(J.seed, J.stream, i, f);
+        atomicAdd(&s_hist[widom_bin(f, J.nb)], 1u);
+    }
+    __syncthreads();
+    for (int b = threadIdx.x; b < nbt; b += kWG)
+        if (s_hist[b]) atomicAdd(&J.count[b], s_hist[b]);
+}
+
+__global__ __launch_bounds__(kWG) void k_bin_scan(const BinJob *__restrict__ jobs) {
+    const BinJob &J = jobs[blockIdx.x];
+    const long long nbt = (long long)J.nb[0] * J.nb[1] * J.nb[2];
+    block_exclusive_scan(J.count, nbt, J.start, J.cursor);
+}
+
+__global__ __launch_bounds__(kWG) void k_bin_scatter(const BinJob *__restrict__ jobs, int njobs,
+                                                     int chunk) {
+    __shared__ uint32_t s_hist[kLdsBins];
+    const int j = bin_job_of(jobs, njobs, blockIdx.x);
+    const BinJob &J = jobs[j];
+    const uint64_t i0 = (uint64_t)(blockIdx.x - J.block0) * (uint64_t)chunk;
+    const uint64_t i1 = i0 + (uint64_t)chunk < J.n ? i0 + (uint64_t)chunk : J.n;
+    const int nbt = J.nb[0] * J.nb[1] * J.nb[2];
+    if (nbt > kLdsBins) {
+        for (uint64_t i = i0 + threadIdx.x; i < i1; i += kWG) {
+            double f[3];
+            widom_point(J.seed, J.stream, i, f);
+            const uint32_t pos = atomicAdd(&J.cursor[widom_bin(f, J.nb)], 1u);
+            J.sorted[pos] = (uint32_t)i;
+        }
+        return;
+    }
+    for (int b = threadIdx.x; b < nbt; b += kWG) s_hist[b] = 0u;
+    __syncthreads();
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kWG) {
+        double f[3];
+        widom_point(J.seed, J.stream, i, f);
+        atomicAdd(&s_hist[widom_bin(f, J.nb)], 1u);
+    }
+    __syncthreads();
+    // reserve each bin's share of slots; s_hist then holds this block's next slot
+    for (int b = threadIdx.x; b < nbt; b += kWG) {
+        const uint32_t h = s_hist[b];
+        if (h) s_hist[b] = atomicAdd(&J.cursor[b], h);
+    }
+    __syncthreads();
+    for (uint64_t i = i0 + threadIdx.x; i < i1; i += kWG) {
+        double f[3];
+        widom_point(J.seed, J.stream, i, f);
+        const uint32_t pos = atomicAdd(&s_hist[widom_bin(f, J.nb)], 1u);
+        J.sorted[pos] = (uint32_t)i;
+    }
 }
 
 __global__ void k_fill_u64(unsigned long long *p, unsigned long long v, long long n) {
@@ -877,12 +940,6 @@ hipError_t launch_near_write(const NearPlan &p, const uint32_t *block_off, doubl
     return hipGetLastError();
 }
 
-hipError_t launch_fill_u32(uint32_t *p, uint32_t v, long long n, hipStream_t s) {
-    if (n <= 0) return hipSuccess;
-    hipLaunchKernelGGL(k_fill_u32, dim3(grid_blocks(n, kWG, 4096)), dim3(kWG), 0, s, p, v, n);
-    return hipGetLastError();
-}
-
 hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, long long n,
                            hipStream_t s) {
     if (n <= 0) return hipSuccess;
@@ -890,25 +947,18 @@ hipError_t launch_fill_u64(unsigned long long *p, unsigned long long v, long lon
     return hipGetLastError();
 }
 
-hipError_t launch_widom_count(uint64_t seed, uint32_t stream, uint64_t n, const int nb[3],
-                              uint32_t *count, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_widom_count, dim3(grid_blocks((long long)n, kWG, 8192)), dim3(kWG), 0, s,
-                       seed, stream, n, nb[0], nb[1], nb[2], count);
-    return hipGetLastError();
-}
+int bin_blocks(uint64_t n, int chunk) { return (int)((n + (uint64_t)chunk - 1) / (uint64_t)chunk); }
 
-hipError_t launch_scan_u32(const uint32_t *count, long long n, uint32_t *start, uint32_t *cursor,
-                           hipStream_t s) {
-    hipLaunchKernelGGL(k_scan_u32, dim3(1), dim3(kWG), 0, s, count, n, start, cursor);
-    return hipGetLastError();
-}
-
-hipError_t launch_widom_scatter(uint64_t seed, uint32_t stream, uint64_t n, const int nb[3],
-                                uint32_t *cursor, uint32_t *sorted, hipStream_t s) {
-    if (n == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_widom_scatter, dim3(grid_blocks((long long)n, kWG, 8192)), dim3(kWG), 0,
-                       s, seed, stream, n, nb[0], nb[1], nb[2], cursor, sorted);
+hipError_t launch_bin_all(const BinJob *jobs, int njobs, long long total_blocks, int chunk,
+                          long long max_bins, hipStream_t s) {
+    if (njobs <= 0 || total_blocks <= 0) return hipSuccess;
+    if (chunk < kWG || total_blocks > (1LL << 31) - 1 || max_bins <= 0) return hipErrorInvalidValue;
+    hipLaunchKernelGGL(k_bin_zero, dim3(njobs), dim3(kWG), 0, s, jobs);
+    hipLaunchKernelGGL(k_bin_count, dim3((unsigned)total_blocks), dim3(kWG), 0, s, jobs, njobs,
+                       chunk);
+    hipLaunchKernelGGL(k_bin_scan, dim3(njobs), dim3(kWG), 0, s, jobs);
+    hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)total_blocks), dim3(kWG), 0, s, jobs, njobs,
+                       chunk);
     return hipGetLastError();
 }
 

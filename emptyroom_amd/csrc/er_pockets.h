@@ -9,20 +9,37 @@
 
 namespace er {
 
-hipError_t launch_cc_init(const float *e, long long nv, double thr, int *parent, hipStream_t s);
-hipError_t launch_cc_hook(const int n[3], int *parent, hipStream_t s);
-hipError_t launch_cc_flatten(long long nv, int *parent, uint32_t *is_root, hipStream_t s);
-hipError_t launch_cc_segments(long long nv, const int *parent, const uint32_t *root_rank,
-                              int *seg, hipStream_t s);
-hipError_t launch_cc_faces(const int n[3], const int *seg, int *links, hipStream_t s);
-hipError_t launch_cc_block(long long nv, const int *seg, const uint8_t *pocket, float *e,
+constexpr int kCCVox = 2048;  // voxels per block of the per-voxel labelling passes
+
+// One FP32 grid of a labelling launch.  Plain-old-data, copied to the device.
+struct CCGrid {
+    float *e;                  // energies, index k + j*nz + i*ny*nz (blocked in place)
+    int n[3];
+    int pad;
+    long long nv;              // voxels
+    long long nw;              // root-mask words, ceil(nv / 64)
+    int *parent;               // nv: union-find parent, then root (-1 inaccessible)
+    unsigned long long *mask;  // nw: bit v set iff voxel v is a root
+    uint32_t *wpre;            // nw + 1: roots before each word; wpre[nw] = segments
+    int *links;                // 3 * faces: compacted (from, to, dir) face links
+    const uint8_t *pocket;     // segments: 1 if pocket (blocking pass), or null
+    long long blk0;            // first block of this grid in per-voxel launches
+    long long seg0;            // offset of this grid in a segment-id output
+};
+
+long long cc_blocks(long long nv);
+// init, hook, flatten, word scan and face links of every grid.  packed
+// receives each grid's links at offset meta[4g+2] (taken from *packed_total,
+// which must start at 0); meta[4g] = links, meta[4g+1] = segments.
+hipError_t launch_cc_label(const CCGrid *grids, int ng, long long total_blocks, double thr,
+                           int *packed, unsigned int *packed_total, unsigned int *meta,
                            hipStream_t s);
+// seg_out[seg0 + v] = segment id of voxel v (-1 inaccessible), after labelling.
+hipError_t launch_cc_segments(const CCGrid *grids, int ng, long long total_blocks, int *seg_out,
+                              hipStream_t s);
+// Overwrite the voxels of pocket segments with 1e13 (grids with pocket != null).
+hipError_t launch_cc_block(const CCGrid *grids, int ng, long long total_blocks, hipStream_t s);
 hipError_t launch_widom_grid(const float *g, const int n[3], uint64_t seed, uint32_t stream,
                              uint64_t nins, double *e_out, hipStream_t s);
-// Exclusive scan of n uint32 values into out[0..n] (out[n] = total) with
-// kScanChunk-element blocks; block_sums needs n/kScanChunk + 2 entries.
-constexpr int kScanChunk = 2048;
-hipError_t launch_scan_large(const uint32_t *in, long long n, uint32_t *out,
-                             uint32_t *block_sums, uint32_t *block_off, hipStream_t s);
 
 }  // namespace er

@@ -1,15 +1,19 @@
 // er_pockets.hip -- pocket blocking (src/pocketblocking.cc:126-391) and
 // grid-interpolated Widom insertion (src/gcmc.cc:145-177) on gfx950.
 //
-// Segments: voxels with E < 15 T, 6-connected, no periodic images.  A
-// lock-free union-find links every accessible voxel to its accessible -x, -y
-// and -z neighbours, always hooking the larger root under the smaller, so the
-// final root of a segment is its smallest linear index k + j*nz + i*ny*nz --
-// the voxel where the reference's raster-order flood fill starts it.  The
-// segment id is then the rank of that root among all roots (an exclusive scan
-// of root flags), which is exactly the reference's numbering.  The face links
-// (x = N-1 -> x = 0 etc.) go to the host, which classifies segments; a last
-// kernel overwrites pocket voxels with 1e13.
+// Segments: voxels with E < 15 T, 6-connected, no periodic images.  Any
+// number of grids run together, one launch per phase, blocks mapped to grids
+// through a table.  A lock-free union-find links every accessible voxel to
+// its accessible -x, -y and -z neighbours, always hooking the larger root
+// under the smaller, so the final root of a segment is its smallest linear
+// index k + j*nz + i*ny*nz -- the voxel where the reference's raster-order
+// flood fill starts it.  The segment id is the rank of that root among all
+// roots; roots are kept as a bitmask (one ballot per 64 voxels) with a
+// prefix count per 64-bit word, so rank(r) = prefix[r/64] +
+// popcount(mask[r/64] below r) and no per-voxel rank array exists.  Face
+// links (x = N-1 -> x = 0 etc.) are run-length compacted on the device and
+// packed into one buffer for the host, which classifies segments; a last
+// pass overwrites pocket voxels with 1e13.
 #include "er_pockets.h"
 
 namespace er {
@@ -65,95 +69,230 @@ __device__ __forceinline__ void unite(int *parent, int a, int b) {
     }
 }
 
-__global__ void k_cc_init(const float *__restrict__ e, long long nv, double thr,
-                          int *__restrict__ parent) {
-    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
-         v += (long long)gridDim.x * blockDim.x)
-        parent[v] = ((double)e[v] >= thr) ? -1 : (int)v;  // src/pocketblocking.cc:138
+// Block -> grid of a per-voxel launch (blocks of kCCVox voxels per grid).
+__device__ __forceinline__ int cc_grid_of(const CCGrid *__restrict__ g, int ng, long long blk) {
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g[mid].blk0 <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
 }
 
-__global__ void k_cc_hook(int nx, int ny, int nz, int *__restrict__ parent) {
-    const long long nv = (long long)nx * ny * nz;
+// Segment id of voxel v after the flatten pass: rank of its root, -1 if
+// inaccessible.
+__device__ __forceinline__ int seg_of(const CCGrid &G, long long v) {
+    const int r = G.parent[v];
+    if (r < 0) return -1;
+    const long long w = r >> 6;
+    const unsigned long long below = G.mask[w] & ((1ull << (r & 63)) - 1ull);
+    return (int)(G.wpre[w] + (uint32_t)__popcll(below));
+}
+
+__global__ __launch_bounds__(kWG) void k_cc_init(const CCGrid *__restrict__ grids, int ng,
+                                                 double thr) {
+    const CCGrid &G = grids[cc_grid_of(grids, ng, blockIdx.x)];
+    const long long base = (long long)(blockIdx.x - G.blk0) * kCCVox;
+    for (int k = threadIdx.x; k < kCCVox; k += kWG) {
+        const long long v = base + k;
+        if (v < G.nv) G.parent[v] = ((double)G.e[v] >= thr) ? -1 : (int)v;  // src/pocketblocking.cc:138
+    }
+}
+
+__global__ __launch_bounds__(kWG) void k_cc_hook(const CCGrid *__restrict__ grids, int ng) {
+    const CCGrid &G = grids[cc_grid_of(grids, ng, blockIdx.x)];
+    const int nz = G.n[2], ny = G.n[1];
     const long long syz = (long long)ny * nz;
-    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
-         v += (long long)gridDim.x * blockDim.x) {
-        if (parent[v] < 0) continue;
-        const int k = (int)(v % nz), j = (int)((v / nz) % ny), i = (int)(v / syz);
-        if (k > 0 && parent[v - 1] >= 0) unite(parent, (int)v, (int)(v - 1));
+    int *parent = G.parent;
+    const long long base = (long long)(blockIdx.x - G.blk0) * kCCVox;
+    for (int k = threadIdx.x; k < kCCVox; k += kWG) {
+        const long long v = base + k;
+        if (v >= G.nv || parent[v] < 0) continue;
+        const int kz = (int)(v % nz), j = (int)((v / nz) % ny), i = (int)(v / syz);
+        if (kz > 0 && parent[v - 1] >= 0) unite(parent, (int)v, (int)(v - 1));
         if (j > 0 && parent[v - nz] >= 0) unite(parent, (int)v, (int)(v - nz));
         if (i > 0 && parent[v - syz] >= 0) unite(parent, (int)v, (int)(v - syz));
     }
 }
 
-__global__ void k_cc_flatten(long long nv, int *__restrict__ parent,
-                             uint32_t *__restrict__ is_root) {
-    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
-         v += (long long)gridDim.x * blockDim.x) {
-        const int p = parent[v];
-        if (p < 0) {
-            is_root[v] = 0;
-            continue;
+// parent[v] = root of v; bit v of mask set iff v is a root.  Each wave covers
+// 64 consecutive voxels starting at a multiple of 64, so one ballot is one
+// mask word.
+__global__ __launch_bounds__(kWG) void k_cc_flatten(const CCGrid *__restrict__ grids, int ng) {
+    const CCGrid &G = grids[cc_grid_of(grids, ng, blockIdx.x)];
+    const long long base = (long long)(blockIdx.x - G.blk0) * kCCVox;
+    for (int k = threadIdx.x; k < kCCVox; k += kWG) {
+        const long long v = base + k;
+        bool root = false;
+        if (v < G.nv) {
+            const int p = G.parent[v];
+            if (p >= 0) {
+                // only v's own thread writes parent[v] here, so it ends as the root
+                const int r = find_root_ro(G.parent, (int)v);
+                st(&G.parent[v], r);
+                root = r == (int)v;
+            }
         }
-        // Only v's own thread writes parent[v] here, so it ends as the root.
-        const int r = find_root_ro(parent, (int)v);
-        st(&parent[v], r);
-        is_root[v] = (r == (int)v) ? 1u : 0u;
+        const unsigned long long m = __ballot(root);
+        const long long w = v >> 6;
+        if ((threadIdx.x & 63) == 0 && w < G.nw) G.mask[w] = m;
     }
 }
 
-// seg[v] = rank of the root of v among all roots, -1 if inaccessible.
-__global__ void k_cc_segments(long long nv, const int *__restrict__ parent,
-                              const uint32_t *__restrict__ root_rank, int *__restrict__ seg) {
-    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
-         v += (long long)gridDim.x * blockDim.x) {
-        const int p = parent[v];
-        seg[v] = p < 0 ? -1 : (int)root_rank[p];
+// wpre[w] = roots in words before w; wpre[nw] = number of segments.  One
+// block per grid.
+__global__ __launch_bounds__(kWG) void k_cc_word_scan(const CCGrid *__restrict__ grids) {
+    const CCGrid &G = grids[blockIdx.x];
+    __shared__ uint32_t s_part[kWG];
+    __shared__ uint32_t s_carry;
+    const int tid = threadIdx.x;
+    if (tid == 0) s_carry = 0;
+    __syncthreads();
+    constexpr int per = 8;
+    for (long long b = 0; b < G.nw; b += (long long)kWG * per) {
+        uint32_t v[per], sum = 0;
+#pragma unroll
+        for (int k = 0; k < per; ++k) {
+            const long long w = b + (long long)tid * per + k;
+            v[k] = w < G.nw ? (uint32_t)__popcll(G.mask[w]) : 0u;
+            sum += v[k];
+        }
+        s_part[tid] = sum;
+        __syncthreads();
+        for (int d = 1; d < kWG; d <<= 1) {
+            const uint32_t add = tid >= d ? s_part[tid - d] : 0u;
+            __syncthreads();
+            s_part[tid] += add;
+            __syncthreads();
+        }
+        uint32_t run = s_carry + s_part[tid] - sum;
+#pragma unroll
+        for (int k = 0; k < per; ++k) {
+            const long long w = b + (long long)tid * per + k;
+            if (w < G.nw) G.wpre[w] = run;
+            run += v[k];
+        }
+        __syncthreads();
+        if (tid == kWG - 1) s_carry += s_part[kWG - 1];
+        __syncthreads();
     }
+    if (tid == 0) G.wpre[G.nw] = s_carry;
 }
 
-// Face links of src/pocketblocking.cc:273-321: for each face pair the
-// segment at index N-1 links to the segment at index 0 in direction +1/+2/+3.
-// links[3*q] = from, [3*q+1] = to, [3*q+2] = direction, or from = -1.
-__global__ void k_cc_faces(int nx, int ny, int nz, const int *__restrict__ seg,
-                           int *__restrict__ links) {
+// Face links of src/pocketblocking.cc:273-321: for each face pair the segment
+// at index N-1 links to the segment at index 0 in direction +1/+2/+3.  Pairs
+// with an inaccessible side are dropped, and so is every pair equal to the
+// one before it in face order (repeats change no classification).  One
+// block per grid writes its compacted (from, to, dir) triples at the start of
+// its region, then copies them to the packed buffer at an offset taken from
+// *packed_total; meta[4g..4g+3] = (links, segments, packed offset, 0).
+__global__ __launch_bounds__(kWG) void k_cc_faces(const CCGrid *__restrict__ grids,
+                                                  int *__restrict__ packed,
+                                                  unsigned int *__restrict__ packed_total,
+                                                  unsigned int *__restrict__ meta) {
+    const CCGrid &G = grids[blockIdx.x];
+    __shared__ int s_a[kWG], s_b[kWG], s_d[kWG];
+    __shared__ int s_cnt[kWG / 64];
+    __shared__ int s_prev[3];
+    __shared__ unsigned int s_base;
+    const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
+    const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
     const long long fx = (long long)ny * nz, fy = (long long)nx * nz, fz = (long long)nx * ny;
     const long long total = fx + fy + fz;
-    for (long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x; q < total;
-         q += (long long)gridDim.x * blockDim.x) {
-        long long here, there;
-        int dir;
-        if (q < fx) {
-            const int j = (int)(q / nz), k = (int)(q % nz);
-            here = (long long)k + (long long)j * nz;
-            there = here + (long long)(nx - 1) * ny * nz;
-            dir = 1;
-        } else if (q < fx + fy) {
-            const long long r = q - fx;
-            const int i = (int)(r / nz), k = (int)(r % nz);
-            here = (long long)k + (long long)i * ny * nz;
-            there = here + (long long)(ny - 1) * nz;
-            dir = 2;
-        } else {
-            const long long r = q - fx - fy;
-            const int i = (int)(r / ny), j = (int)(r % ny);
-            here = (long long)j * nz + (long long)i * ny * nz;
-            there = here + (nz - 1);
-            dir = 3;
+    if (tid == 0) s_prev[0] = s_prev[1] = s_prev[2] = -2;
+    int cnt = 0;
+    __syncthreads();
+    for (long long q0 = 0; q0 < total; q0 += kWG) {
+        const long long q = q0 + tid;
+        int a = -1, b = -1, dir = 0;
+        if (q < total) {
+            long long here, there;
+            if (q < fx) {
+                here = (q / nz) * nz + (q % nz);
+                there = here + (long long)(nx - 1) * ny * nz;
+                dir = 1;
+            } else if (q < fx + fy) {
+                const long long r = q - fx;
+                here = (r % nz) + (r / nz) * (long long)ny * nz;
+                there = here + (long long)(ny - 1) * nz;
+                dir = 2;
+            } else {
+                const long long r = q - fx - fy;
+                here = (r % ny) * nz + (r / ny) * (long long)ny * nz;
+                there = here + (nz - 1);
+                dir = 3;
+            }
+            a = seg_of(G, there);
+            b = seg_of(G, here);
         }
-        const int a = seg[there], b = seg[here];
-        links[3 * q] = (a >= 0 && b >= 0) ? a : -1;
-        links[3 * q + 1] = b;
-        links[3 * q + 2] = dir;
+        s_a[tid] = a;
+        s_b[tid] = b;
+        s_d[tid] = dir;
+        __syncthreads();
+        const int pa = tid > 0 ? s_a[tid - 1] : s_prev[0];
+        const int pb = tid > 0 ? s_b[tid - 1] : s_prev[1];
+        const int pd = tid > 0 ? s_d[tid - 1] : s_prev[2];
+        const bool keep = q < total && a >= 0 && b >= 0 && !(a == pa && b == pb && dir == pd);
+        const unsigned long long bk = __ballot(keep);
+        if (lane == 0) s_cnt[wave] = __popcll(bk);
+        __syncthreads();
+        int off = cnt, tot = 0;
+#pragma unroll
+        for (int w = 0; w < kWG / 64; ++w) {
+            if (w < wave) off += s_cnt[w];
+            tot += s_cnt[w];
+        }
+        if (keep) {
+            const int o = off + __builtin_amdgcn_mbcnt_hi((uint32_t)(bk >> 32),
+                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bk, 0u));
+            G.links[3 * o] = a;
+            G.links[3 * o + 1] = b;
+            G.links[3 * o + 2] = dir;
+        }
+        cnt += tot;
+        __syncthreads();
+        if (tid == kWG - 1) {
+            s_prev[0] = a;
+            s_prev[1] = b;
+            s_prev[2] = dir;
+        }
+        __syncthreads();
+    }
+    // the block's own link writes are read back by every wave below
+    __threadfence();
+    if (tid == 0) s_base = atomicAdd(packed_total, (unsigned int)cnt);
+    __syncthreads();
+    const unsigned int base = s_base;
+    for (int k = tid; k < 3 * cnt; k += kWG) packed[3 * (long long)base + k] = G.links[k];
+    if (tid == 0) {
+        meta[4 * blockIdx.x] = (unsigned int)cnt;
+        meta[4 * blockIdx.x + 1] = G.wpre[G.nw];
+        meta[4 * blockIdx.x + 2] = base;
+        meta[4 * blockIdx.x + 3] = 0u;
     }
 }
 
-// src/pocketblocking.cc:373-382 on an FP32 grid.
-__global__ void k_cc_block(long long nv, const int *__restrict__ seg,
-                           const uint8_t *__restrict__ pocket, float *__restrict__ e) {
-    for (long long v = (long long)blockIdx.x * blockDim.x + threadIdx.x; v < nv;
-         v += (long long)gridDim.x * blockDim.x) {
-        const int s = seg[v];
-        if (s >= 0 && pocket[s]) e[v] = 1e13f;
+__global__ __launch_bounds__(kWG) void k_cc_segments(const CCGrid *__restrict__ grids, int ng,
+                                                     int *__restrict__ seg_out) {
+    const CCGrid &G = grids[cc_grid_of(grids, ng, blockIdx.x)];
+    const long long base = (long long)(blockIdx.x - G.blk0) * kCCVox;
+    for (int k = threadIdx.x; k < kCCVox; k += kWG) {
+        const long long v = base + k;
+        if (v < G.nv) seg_out[G.seg0 + v] = seg_of(G, v);
+    }
+}
+
+// src/pocketblocking.cc:373-382 on FP32 grids.
+__global__ __launch_bounds__(kWG) void k_cc_block(const CCGrid *__restrict__ grids, int ng) {
+    const CCGrid &G = grids[cc_grid_of(grids, ng, blockIdx.x)];
+    if (!G.pocket) return;
+    const long long base = (long long)(blockIdx.x - G.blk0) * kCCVox;
+    for (int k = threadIdx.x; k < kCCVox; k += kWG) {
+        const long long v = base + k;
+        if (v >= G.nv) continue;
+        const int s = seg_of(G, v);
+        if (s >= 0 && G.pocket[s]) G.e[v] = 1e13f;
     }
 }
 
@@ -198,60 +337,6 @@ __global__ void k_widom_grid(const float *__restrict__ g, int nx, int ny, int nz
     }
 }
 
-__global__ __launch_bounds__(256) void k_scan_block_sums(const uint32_t *in, long long n,
-                                                       uint32_t *sums) {
-    __shared__ uint32_t s_w[4];
-    const long long base = (long long)blockIdx.x * kScanChunk;
-    uint32_t v = 0;
-    for (int k = threadIdx.x; k < kScanChunk; k += 256) {
-        const long long i = base + k;
-        if (i < n) v += in[i];
-    }
-    for (int o = 32; o >= 1; o >>= 1) v += __shfl_xor(v, o, 64);
-    if ((threadIdx.x & 63) == 0) s_w[threadIdx.x >> 6] = v;
-    __syncthreads();
-    if (threadIdx.x == 0) sums[blockIdx.x] = s_w[0] + s_w[1] + s_w[2] + s_w[3];
-}
-
-__global__ __launch_bounds__(256) void k_scan_sums(const uint32_t *sums, int nb, uint32_t *off) {
-    if (threadIdx.x != 0) return;
-    uint32_t run = 0;
-    for (int b = 0; b < nb; ++b) {
-        off[b] = run;
-        run += sums[b];
-    }
-    off[nb] = run;
-}
-
-__global__ __launch_bounds__(256) void k_scan_apply(const uint32_t *in, long long n,
-                                                  const uint32_t *off, uint32_t *out, int nb) {
-    __shared__ uint32_t s_part[256];
-    const long long base = (long long)blockIdx.x * kScanChunk;
-    const int per = kScanChunk / 256;
-    uint32_t v[kScanChunk / 256];
-    uint32_t sum = 0;
-    for (int k = 0; k < per; ++k) {
-        const long long i = base + (long long)threadIdx.x * per + k;
-        v[k] = i < n ? in[i] : 0u;
-        sum += v[k];
-    }
-    s_part[threadIdx.x] = sum;
-    __syncthreads();
-    for (int d = 1; d < 256; d <<= 1) {
-        const uint32_t add = threadIdx.x >= d ? s_part[threadIdx.x - d] : 0u;
-        __syncthreads();
-        s_part[threadIdx.x] += add;
-        __syncthreads();
-    }
-    uint32_t run = off[blockIdx.x] + s_part[threadIdx.x] - sum;
-    for (int k = 0; k < per; ++k) {
-        const long long i = base + (long long)threadIdx.x * per + k;
-        if (i < n) out[i] = run;
-        run += v[k];
-    }
-    if (blockIdx.x == 0 && threadIdx.x == 0) out[n] = off[nb];
-}
-
 int blocks_for(long long n) {
     long long b = (n + 255) / 256;
     return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
@@ -259,51 +344,33 @@ int blocks_for(long long n) {
 
 }  // namespace
 
-hipError_t launch_cc_init(const float *e, long long nv, double thr, int *parent, hipStream_t s) {
-    hipLaunchKernelGGL(k_cc_init, dim3(blocks_for(nv)), dim3(256), 0, s, e, nv, thr, parent);
-    return hipGetLastError();
-}
+long long cc_blocks(long long nv) { return (nv + kCCVox - 1) / kCCVox; }
 
-hipError_t launch_cc_hook(const int n[3], int *parent, hipStream_t s) {
-    const long long nv = (long long)n[0] * n[1] * n[2];
-    hipLaunchKernelGGL(k_cc_hook, dim3(blocks_for(nv)), dim3(256), 0, s, n[0], n[1], n[2],
-                       parent);
-    return hipGetLastError();
-}
-
-hipError_t launch_cc_flatten(long long nv, int *parent, uint32_t *is_root, hipStream_t s) {
-    hipLaunchKernelGGL(k_cc_flatten, dim3(blocks_for(nv)), dim3(256), 0, s, nv, parent, is_root);
-    return hipGetLastError();
-}
-
-hipError_t launch_cc_segments(long long nv, const int *parent, const uint32_t *root_rank,
-                              int *seg, hipStream_t s) {
-    hipLaunchKernelGGL(k_cc_segments, dim3(blocks_for(nv)), dim3(256), 0, s, nv, parent,
-                       root_rank, seg);
-    return hipGetLastError();
-}
-
-hipError_t launch_cc_faces(const int n[3], const int *seg, int *links, hipStream_t s) {
-    const long long total = (long long)n[1] * n[2] + (long long)n[0] * n[2] + (long long)n[0] * n[1];
-    hipLaunchKernelGGL(k_cc_faces, dim3(blocks_for(total)), dim3(256), 0, s, n[0], n[1], n[2],
-                       seg, links);
-    return hipGetLastError();
-}
-
-hipError_t launch_cc_block(long long nv, const int *seg, const uint8_t *pocket, float *e,
+hipError_t launch_cc_label(const CCGrid *grids, int ng, long long total_blocks, double thr,
+                           int *packed, unsigned int *packed_total, unsigned int *meta,
                            hipStream_t s) {
-    hipLaunchKernelGGL(k_cc_block, dim3(blocks_for(nv)), dim3(256), 0, s, nv, seg, pocket, e);
+    if (ng <= 0) return hipSuccess;
+    if (total_blocks <= 0 || total_blocks > (1LL << 31) - 1) return hipErrorInvalidValue;
+    const dim3 nb((unsigned)total_blocks);
+    hipLaunchKernelGGL(k_cc_init, nb, dim3(kWG), 0, s, grids, ng, thr);
+    hipLaunchKernelGGL(k_cc_hook, nb, dim3(kWG), 0, s, grids, ng);
+    hipLaunchKernelGGL(k_cc_flatten, nb, dim3(kWG), 0, s, grids, ng);
+    hipLaunchKernelGGL(k_cc_word_scan, dim3(ng), dim3(kWG), 0, s, grids);
+    hipLaunchKernelGGL(k_cc_faces, dim3(ng), dim3(kWG), 0, s, grids, packed, packed_total, meta);
     return hipGetLastError();
 }
 
-hipError_t launch_scan_large(const uint32_t *in, long long n, uint32_t *out,
-                             uint32_t *block_sums, uint32_t *block_off, hipStream_t s) {
-    const long long nb = (n + kScanChunk - 1) / kScanChunk;
-    if (nb <= 0 || nb > (1LL << 30)) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(k_scan_block_sums, dim3((unsigned)nb), dim3(256), 0, s, in, n, block_sums);
-    hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(64), 0, s, block_sums, (int)nb, block_off);
-    hipLaunchKernelGGL(k_scan_apply, dim3((unsigned)nb), dim3(256), 0, s, in, n, block_off, out,
-                       (int)nb);
+hipError_t launch_cc_segments(const CCGrid *grids, int ng, long long total_blocks, int *seg_out,
+                              hipStream_t s) {
+    if (ng <= 0 || total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cc_segments, dim3((unsigned)total_blocks), dim3(kWG), 0, s, grids, ng,
+                       seg_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_cc_block(const CCGrid *grids, int ng, long long total_blocks, hipStream_t s) {
+    if (ng <= 0 || total_blocks <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_cc_block, dim3((unsigned)total_blocks), dim3(kWG), 0, s, grids, ng);
     return hipGetLastError();
 }
 

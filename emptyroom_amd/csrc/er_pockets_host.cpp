@@ -1,11 +1,15 @@
 // er_pockets_host.cpp -- pocket blocking and grid-interpolated Widom on
 // resident FP32 grids (src/pocketblocking.cc:126-391, src/gcmc.cc:145-177).
+#include <algorithm>
 #include <array>
+#include <atomic>
 #include <cmath>
 #include <cstring>
 #include <sstream>
+#include <thread>
 #include <vector>
 
+#include "er_kernels.h"
 #include "er_pockets.h"
 #include "er_runtime.h"
 
@@ -23,6 +27,12 @@ namespace {
             set_error(os_.str());                                                          \
             return ST_HIP;                                                                 \
         }                                                                                  \
+    } while (0)
+
+#define ST_TRY_P(expr)              \
+    do {                            \
+        int st_ = (expr);           \
+        if (st_ != ST_OK) return st_; \
     } while (0)
 
 struct Scratch {
@@ -149,65 +159,183 @@ int grid_upload(Context *ctx, const float *host, const int n[3], GridDev **out) 
     return ST_OK;
 }
 
-int pockets_label(Context *ctx, GridDev *g, double temperature, bool block, int *n_pockets,
-                  int32_t *seg_host) {
-    if (!ctx || !g || !n_pockets) {
-        set_error("pockets: null argument");
-        return ST_ARG;
+namespace {
+
+// Voxels labelled per launch sequence (about 4.3 B of scratch per voxel).
+constexpr long long kCCChunkVoxels = 1LL << 28;
+
+int label_chunk(Context *ctx, GridDev *const *grids, int ng, double temperature, bool block,
+                int *n_pockets, int32_t *const *seg_host) {
+    hipStream_t s = ctx->stream;
+    Scratch sc{ctx, {}};
+    std::vector<CCGrid> tab(ng);
+    std::vector<long long> faces(ng);
+    long long nv_sum = 0, nw_sum = 0, face_sum = 0, blocks = 0;
+    for (int g = 0; g < ng; g++) {
+        const int *n = grids[g]->n;
+        CCGrid &c = tab[g];
+        std::memset(&c, 0, sizeof(c));
+        c.e = (float *)grids[g]->d_data;
+        std::memcpy(c.n, n, sizeof(c.n));
+        c.nv = (long long)n[0] * n[1] * n[2];
+        c.nw = (c.nv + 63) / 64;
+        c.blk0 = blocks;
+        c.seg0 = nv_sum;
+        faces[g] = (long long)n[1] * n[2] + (long long)n[0] * n[2] + (long long)n[0] * n[1];
+        blocks += cc_blocks(c.nv);
+        nv_sum += c.nv;
+        nw_sum += c.nw + 1;
+        face_sum += faces[g];
     }
-    if (g->precision != 0) {
-        set_error("pockets: the grid must be FP32");
+    int *parent = sc.get<int>(nv_sum), *links = sc.get<int>(3 * face_sum),
+        *packed = sc.get<int>(3 * face_sum);
+    unsigned long long *mask = sc.get<unsigned long long>(nw_sum);
+    uint32_t *wpre = sc.get<uint32_t>(nw_sum);
+    unsigned long long *total = sc.get<unsigned long long>(1);
+    unsigned int *meta = sc.get<unsigned int>(4 * (size_t)ng);
+    CCGrid *d_tab = sc.get<CCGrid>(ng);
+    if (!parent || !links || !packed || !mask || !wpre || !total || !meta || !d_tab)
+        return ST_NOMEM;
+    long long ov = 0, ow = 0, of = 0;
+    for (int g = 0; g < ng; g++) {
+        tab[g].parent = parent + ov;
+        tab[g].mask = mask + ow;
+        tab[g].wpre = wpre + ow;
+        tab[g].links = links + 3 * of;
+        ov += tab[g].nv;
+        ow += tab[g].nw + 1;
+        of += faces[g];
+    }
+    HIP_TRY(hipMemcpyAsync(d_tab, tab.data(), sizeof(CCGrid) * ng, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_fill_u64(total, 0ull, 1, s));
+    // src/pocketblocking.cc:130: E >= 15 T is inaccessible
+    HIP_TRY(launch_cc_label(d_tab, ng, blocks, 15.0 * temperature, packed,
+                            (unsigned int *)total, meta, s));
+    std::vector<unsigned int> hmeta(4 * (size_t)ng);
+    HIP_TRY(hipMemcpyAsync(hmeta.data(), meta, sizeof(unsigned int) * 4 * ng,
+                           hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    long long nlinks = 0;
+    for (int g = 0; g < ng; g++) nlinks += hmeta[4 * g];
+    std::vector<int> hl(3 * (size_t)std::max<long long>(nlinks, 1));
+    if (nlinks > 0) {
+        HIP_TRY(hipMemcpyAsync(hl.data(), packed, sizeof(int) * 3 * nlinks,
+                               hipMemcpyDeviceToHost, s));
+        HIP_TRY(hipStreamSynchronize(s));
+    }
+    // classify every grid (independent; spread over host threads)
+    std::vector<std::vector<uint8_t>> pocket(ng);
+    std::vector<int> np(ng, 0);
+    auto work = [&](int g) {
+        np[g] = classify_pockets((int)hmeta[4 * g + 1], hl.data() + 3 * (size_t)hmeta[4 * g + 2],
+                                 hmeta[4 * g], &pocket[g]);
+    };
+    const int nthreads = std::min<int>(ng, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    if (nthreads <= 1) {
+        for (int g = 0; g < ng; g++) work(g);
+    } else {
+        std::vector<std::thread> pool;
+        std::atomic<int> next{0};
+        for (int t = 0; t < nthreads; t++)
+            pool.emplace_back([&] {
+                for (int g = next++; g < ng; g = next++) work(g);
+            });
+        for (auto &th : pool) th.join();
+    }
+    for (int g = 0; g < ng; g++)
+        if (np[g] < 0) {
+            set_error("pockets: inconsistent face links");
+            return ST_STATE;
+        }
+    bool any_seg = false;
+    for (int g = 0; g < ng && seg_host; g++) any_seg |= seg_host[g] != nullptr;
+    long long pk_sum = 0;
+    for (int g = 0; g < ng; g++)
+        if (block && np[g] > 0) pk_sum += (long long)pocket[g].size();
+    if (pk_sum > 0) {
+        std::vector<uint8_t> hp(pk_sum);
+        uint8_t *d_pocket = sc.get<uint8_t>(pk_sum);
+        if (!d_pocket) return ST_NOMEM;
+        long long o = 0;
+        for (int g = 0; g < ng; g++) {
+            tab[g].pocket = nullptr;
+            if (!(block && np[g] > 0)) continue;
+            std::memcpy(hp.data() + o, pocket[g].data(), pocket[g].size());
+            tab[g].pocket = d_pocket + o;
+            o += (long long)pocket[g].size();
+        }
+        HIP_TRY(hipMemcpyAsync(d_pocket, hp.data(), pk_sum, hipMemcpyHostToDevice, s));
+        // blocking changes energies only; segment ids come from parent / mask
+        HIP_TRY(hipMemcpyAsync(d_tab, tab.data(), sizeof(CCGrid) * ng, hipMemcpyHostToDevice, s));
+        HIP_TRY(launch_cc_block(d_tab, ng, blocks, s));
+    }
+    if (any_seg) {
+        int *seg = sc.get<int>(nv_sum);
+        if (!seg) return ST_NOMEM;
+        HIP_TRY(launch_cc_segments(d_tab, ng, blocks, seg, s));
+        for (int g = 0; g < ng; g++)
+            if (seg_host[g])
+                HIP_TRY(hipMemcpyAsync(seg_host[g], seg + tab[g].seg0, sizeof(int) * tab[g].nv,
+                                       hipMemcpyDeviceToHost, s));
+    }
+    HIP_TRY(hipStreamSynchronize(s));
+    for (int g = 0; g < ng; g++) n_pockets[g] = np[g];
+    return ST_OK;
+}
+
+}  // namespace
+
+int pockets_label_many(Context *ctx, GridDev *const *grids, int ng, double temperature,
+                       bool block, int *n_pockets, int32_t *const *seg_host) {
+    if (!ctx || !grids || !n_pockets || ng < 0) {
+        set_error("pockets: null argument");
         return ST_ARG;
     }
     if (!(temperature > 0.0)) {
         set_error("pockets: temperature must be positive");
         return ST_ARG;
     }
-    const long long nv = (long long)g->n[0] * g->n[1] * g->n[2];
-    if (nv >= (1LL << 31) - 1) {
-        set_error("pockets: grid too large for 32-bit voxel indices");
-        return ST_ARG;
+    for (int g = 0; g < ng; g++) {
+        if (!grids[g]) {
+            set_error("pockets: null grid");
+            return ST_ARG;
+        }
+        if (grids[g]->precision != 0) {
+            set_error("pockets: the grid must be FP32");
+            return ST_ARG;
+        }
+        const long long nv = (long long)grids[g]->n[0] * grids[g]->n[1] * grids[g]->n[2];
+        if (nv >= (1LL << 31) - 1) {
+            set_error("pockets: grid too large for 32-bit voxel indices");
+            return ST_ARG;
+        }
     }
     HIP_TRY(hipSetDevice(ctx->device));
-    Scratch sc{ctx, {}};
-    const long long nfaces = (long long)g->n[1] * g->n[2] + (long long)g->n[0] * g->n[2] +
-                             (long long)g->n[0] * g->n[1];
-    const long long nblk = (nv + kScanChunk - 1) / kScanChunk;
-    int *parent = sc.get<int>(nv), *seg = sc.get<int>(nv), *links = sc.get<int>(3 * nfaces);
-    uint32_t *is_root = sc.get<uint32_t>(nv), *rank = sc.get<uint32_t>(nv + 1);
-    uint32_t *bsum = sc.get<uint32_t>(nblk + 2), *boff = sc.get<uint32_t>(nblk + 2);
-    if (!parent || !seg || !links || !is_root || !rank || !bsum || !boff) return ST_NOMEM;
-    hipStream_t s = ctx->stream;
-    const float *e = (const float *)g->d_data;
-    // src/pocketblocking.cc:130: E >= 15 T is inaccessible
-    HIP_TRY(launch_cc_init(e, nv, 15.0 * temperature, parent, s));
-    HIP_TRY(launch_cc_hook(g->n, parent, s));
-    HIP_TRY(launch_cc_flatten(nv, parent, is_root, s));
-    HIP_TRY(launch_scan_large(is_root, nv, rank, bsum, boff, s));
-    HIP_TRY(launch_cc_segments(nv, parent, rank, seg, s));
-    HIP_TRY(launch_cc_faces(g->n, seg, links, s));
-    std::vector<int> hl(3 * nfaces);
-    uint32_t nseg = 0;
-    HIP_TRY(hipMemcpyAsync(hl.data(), links, sizeof(int) * 3 * nfaces, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipMemcpyAsync(&nseg, rank + nv, sizeof(uint32_t), hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    std::vector<uint8_t> pocket;
-    const int np = classify_pockets((int)nseg, hl.data(), nfaces, &pocket);
-    if (np < 0) {
-        set_error("pockets: inconsistent face links");
-        return ST_STATE;
+    int g0 = 0;
+    while (g0 < ng) {
+        // grids of this chunk: at least one, then up to the voxel budget
+        long long nv = 0;
+        int g1 = g0;
+        do {
+            nv += (long long)grids[g1]->n[0] * grids[g1]->n[1] * grids[g1]->n[2];
+            g1++;
+        } while (g1 < ng && nv + (long long)grids[g1]->n[0] * grids[g1]->n[1] * grids[g1]->n[2] <=
+                                kCCChunkVoxels);
+        ST_TRY_P(label_chunk(ctx, grids + g0, g1 - g0, temperature, block, n_pockets + g0,
+                             seg_host ? seg_host + g0 : nullptr));
+        g0 = g1;
     }
-    if (block && np > 0) {
-        uint8_t *d_pocket = sc.get<uint8_t>(pocket.size());
-        if (!d_pocket) return ST_NOMEM;
-        HIP_TRY(hipMemcpyAsync(d_pocket, pocket.data(), pocket.size(), hipMemcpyHostToDevice, s));
-        HIP_TRY(launch_cc_block(nv, seg, d_pocket, (float *)g->d_data, s));
-    }
-    if (seg_host)
-        HIP_TRY(hipMemcpyAsync(seg_host, seg, sizeof(int) * nv, hipMemcpyDeviceToHost, s));
-    HIP_TRY(hipStreamSynchronize(s));
-    *n_pockets = np;
     return ST_OK;
+}
+
+int pockets_label(Context *ctx, GridDev *g, double temperature, bool block, int *n_pockets,
+                  int32_t *seg_host) {
+    if (!g) {
+        set_error("pockets: null argument");
+        return ST_ARG;
+    }
+    int32_t *segs[1] = {seg_host};
+    return pockets_label_many(ctx, &g, 1, temperature, block, n_pockets, segs);
 }
 
 int widom_grid_run(Context *ctx, GridDev *g, uint64_t n_ins, double temperature, uint64_t seed,

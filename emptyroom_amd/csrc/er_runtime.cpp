@@ -581,17 +581,32 @@ int widom_prepare(Context *ctx, FrameworkDev *fw, uint64_t n_ins, WidomBuffers *
     ST_TRY(pool_alloc(ctx, nb + 1, &wb->cursor, owned));
     ST_TRY(pool_alloc(ctx, n_ins, &wb->sorted, owned));
     ST_TRY(pool_alloc(ctx, n_ins, &wb->e, owned));
+    ST_TRY(pool_alloc(ctx, 1, &wb->d_job, owned));
     return ST_OK;
 }
 
-int widom_enqueue_binning(Context *ctx, const WidomBuffers &wb, uint64_t n_ins, uint64_t seed,
+BinJob make_bin_job(const WidomBuffers &wb, uint64_t n_ins, uint64_t seed, uint32_t stream_id) {
+    BinJob j;
+    std::memset(&j, 0, sizeof(j));
+    j.seed = seed;
+    j.n = n_ins;
+    j.stream = stream_id;
+    for (int d = 0; d < 3; d++) j.nb[d] = wb.nbins[d];
+    j.count = wb.count;
+    j.start = wb.start;
+    j.cursor = wb.cursor;
+    j.sorted = wb.sorted;
+    j.block0 = 0;
+    return j;
+}
+
+int widom_enqueue_binning(Context *ctx, WidomBuffers &wb, uint64_t n_ins, uint64_t seed,
                           uint32_t stream_id, hipStream_t s) {
-    const long long nb = (long long)wb.nbins[0] * wb.nbins[1] * wb.nbins[2];
     (void)ctx;
-    HIP_TRY(launch_fill_u32(wb.count, 0u, nb + 1, s));
-    HIP_TRY(launch_widom_count(seed, stream_id, n_ins, wb.nbins, wb.count, s));
-    HIP_TRY(launch_scan_u32(wb.count, nb, wb.start, wb.cursor, s));
-    HIP_TRY(launch_widom_scatter(seed, stream_id, n_ins, wb.nbins, wb.cursor, wb.sorted, s));
+    const long long nb = (long long)wb.nbins[0] * wb.nbins[1] * wb.nbins[2];
+    wb.job = make_bin_job(wb, n_ins, seed, stream_id);
+    HIP_TRY(hipMemcpyAsync(wb.d_job, &wb.job, sizeof(BinJob), hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_bin_all(wb.d_job, 1, bin_blocks(n_ins, kBinChunk), kBinChunk, nb, s));
     return ST_OK;
 }
 
@@ -789,9 +804,9 @@ double job_cost_per_item(const FrameworkDev *fw, const JobDev &j, const Tuning &
 }  // namespace
 
 int Batch::record(hipStream_t s) {
-    for (size_t t = 0; t < tasks.size(); t++)
-        if (tasks[t].n_ins > 0)
-            ST_TRY(widom_enqueue_binning(ctx, wbufs[t], tasks[t].n_ins, tasks[t].seed, 0u, s));
+    // every Widom task's binning in one launch per phase
+    HIP_TRY(launch_bin_all(d_bin_jobs, (int)bin_jobs.size(), bin_blocks_total, kBinChunk,
+                           std::max<long long>(bin_max, 1), s));
     // jobs are partitioned: [0, n_mixed) mixed math, [n_mixed, njobs) FP64 grids
     int n_mixed = 0;
     long long items_mixed = 0, items_fp64 = 0;
@@ -886,6 +901,12 @@ int batch_create(Context *ctx, const Task *tasks, int n, bool use_graph, Batch *
             }
             int st = widom_prepare(ctx, tk.fw, tk.n_ins, &b->wbufs[t], &b->owned);
             if (st) return fail(st);
+            BinJob bj = make_bin_job(b->wbufs[t], tk.n_ins, tk.seed, 0u);
+            bj.block0 = b->bin_blocks_total;
+            b->bin_blocks_total += bin_blocks(tk.n_ins, kBinChunk);
+            b->bin_max = std::max<long long>(
+                b->bin_max, (long long)bj.nb[0] * bj.nb[1] * bj.nb[2]);
+            b->bin_jobs.push_back(bj);
             JobDev j = make_widom_job(tk.fw, tk.n_ins, tk.seed, b->wbufs[t]);
             mixed.push_back({job_cost_per_item(tk.fw, j, ctx->tuning), j});
         }
@@ -912,15 +933,20 @@ int batch_create(Context *ctx, const Task *tasks, int n, bool use_graph, Batch *
     if ((st = pool_alloc(ctx, std::max<size_t>(b->jobs.size(), 1), &b->d_jobs, &b->owned)) ||
         (st = pool_alloc(ctx, (size_t)std::max(n, 1) * kReduceBlocks, &b->d_partials, &b->owned)) ||
         (st = pool_alloc(ctx, (size_t)std::max(n, 1) * 2, &b->d_sums, &b->owned)) ||
-        (st = pool_alloc(ctx, 2, &b->d_counter, &b->owned)))
+        (st = pool_alloc(ctx, 2, &b->d_counter, &b->owned)) ||
+        (st = pool_alloc(ctx, std::max<size_t>(b->bin_jobs.size(), 1), &b->d_bin_jobs,
+                         &b->owned)))
         return fail(st);
     if (hipHostMalloc((void **)&b->h_sums, sizeof(double) * 2 * std::max(n, 1)) != hipSuccess) {
         set_error("batch_create: pinned host allocation failed");
         return fail(ST_NOMEM);
     }
-    if (!b->jobs.empty() &&
-        hipMemcpyAsync(b->d_jobs, b->jobs.data(), sizeof(JobDev) * b->jobs.size(),
-                       hipMemcpyHostToDevice, ctx->stream) != hipSuccess) {
+    if ((!b->jobs.empty() &&
+         hipMemcpyAsync(b->d_jobs, b->jobs.data(), sizeof(JobDev) * b->jobs.size(),
+                        hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
+        (!b->bin_jobs.empty() &&
+         hipMemcpyAsync(b->d_bin_jobs, b->bin_jobs.data(), sizeof(BinJob) * b->bin_jobs.size(),
+                        hipMemcpyHostToDevice, ctx->stream) != hipSuccess)) {
         set_error("batch_create: job upload failed");
         return fail(ST_HIP);
     }

@@ -148,7 +148,10 @@ struct WidomBuffers {
     int nbins[3] = {1, 1, 1};
     uint32_t *count = nullptr, *start = nullptr, *cursor = nullptr, *sorted = nullptr;
     double *e = nullptr;
+    BinJob job{};               // host copy of the binning job (source of the upload)
+    BinJob *d_job = nullptr;    // device copy
 };
+BinJob make_bin_job(const WidomBuffers &wb, uint64_t n_ins, uint64_t seed, uint32_t stream_id);
 void choose_widom_bins(const FrameworkDev *fw, uint64_t n_ins, const Tuning &tu, int nb[3]);
 void choose_grid_tile(const FrameworkDev *fw, const int n[3], const Tuning &tu, int tile[3]);
 void choose_grid_region(const FrameworkDev *fw, const int n[3], const int tile[3],
@@ -161,7 +164,7 @@ JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
 int widom_prepare(Context *ctx, FrameworkDev *fw, uint64_t n_ins, WidomBuffers *wb,
                   std::vector<void *> *owned);
 // Enqueues point generation, binning and the bin-sorted index list on s.
-int widom_enqueue_binning(Context *ctx, const WidomBuffers &wb, uint64_t n_ins, uint64_t seed,
+int widom_enqueue_binning(Context *ctx, WidomBuffers &wb, uint64_t n_ins, uint64_t seed,
                           uint32_t stream_id, hipStream_t s);
 // Upper bound on the near-list records framework_upload allocates.
 size_t framework_near_capacity(int n_atoms, const double t[9], const int rep[3], double rc2);
@@ -169,6 +172,10 @@ size_t framework_near_capacity(int n_atoms, const double t[9], const int rep[3],
 // Config 5: FP32 grids uploaded from the host, pocket blocking
 // (src/pocketblocking.cc:126-391) and grid-interpolated Widom (src/gcmc.cc:145-177).
 int grid_upload(Context *ctx, const float *host, const int n[3], GridDev **out);
+// Labels (and with block, blocks) many grids together; seg_host (nullable
+// array, nullable entries) receives per-grid segment ids.
+int pockets_label_many(Context *ctx, GridDev *const *grids, int ng, double temperature,
+                       bool block, int *n_pockets, int32_t *const *seg_host);
 int pockets_label(Context *ctx, GridDev *g, double temperature, bool block, int *n_pockets,
                   int32_t *seg_host);
 int widom_grid_run(Context *ctx, GridDev *g, uint64_t n_ins, double temperature, uint64_t seed,
@@ -196,6 +203,9 @@ class Batch {
     std::vector<WidomBuffers> wbufs;   // per task
     std::vector<JobDev> jobs;          // all jobs, cost-ordered
     JobDev *d_jobs = nullptr;
+    std::vector<BinJob> bin_jobs;      // Widom binning of every task with insertions
+    BinJob *d_bin_jobs = nullptr;
+    long long bin_blocks_total = 0, bin_max = 0;
     long long total_items = 0;
     int math = MATH_MIXED;
     double2 *d_partials = nullptr;     // kReduceBlocks per task

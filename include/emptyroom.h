@@ -193,6 +193,12 @@ int er_batch_kernel_ms(er_batch *batch, double *ms, int *launches);
 int er_grid_upload(er_context *ctx, const float *host, const int n[3], er_grid **out);
 int er_pockets_label(er_context *ctx, er_grid *grid, double temperature, int block_in_place,
                      int *n_pockets, int32_t *segments_out);
+/* Many grids labelled together (one launch per phase for all of them);
+ * n_pockets[i] receives grid i's pocket count.  Same results as
+ * er_pockets_label on each grid; no reference counterpart as a batch (the
+ * reference labels one grid per FindAndBlockPockets call). */
+int er_pockets_label_batch(er_context *ctx, er_grid *const *grids, int n_grids,
+                           double temperature, int block_in_place, int *n_pockets);
 int er_widom_grid(er_context *ctx, er_grid *grid, uint64_t n_insertions, double temperature,
                   uint64_t seed, er_widom_result *out, double *energies_out);
 
@@ -284,6 +290,17 @@ void er_host_run_release(er_host_run *run);
  * exactly one of e64 / e32 is non-null. */
 int er_write_grid_file(const char *path, const char *format, const double *e64, const float *e32,
                        const int n[3], const double t_matrix[9], int threads);
+
+/* Replication factors (image half-ranges) of julia_codes/framework.jl:162-211
+ * for a row-major fractional->Cartesian matrix: per axis, 1 plus the number
+ * of cell-vector doublings until the supercell centre is at least `cutoff`
+ * (twice: 2 cutoff) from the face plane of the other two vectors. */
+int er_replication_factors(const double t_matrix[9], double cutoff, int twice, int rep[3]);
+/* Reads data/structures/<structure>.cssr under root and writes
+ * data/uc_replications/<structure>_<once|twice>.uc ("%d %d %d", as
+ * julia_codes/framework.jl:213-227); rep (nullable) receives the factors. */
+int er_write_uc_file(const char *root, const char *structure, double cutoff, int twice,
+                     int rep[3]);
 
 #ifdef __cplusplus
 }

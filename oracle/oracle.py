@@ -202,3 +202,21 @@ def widom_grid(grid, temperature, seed, i0, i1, energies=False, nthreads=0):
 
 def have_ref() -> bool:
     return os.path.exists(REF_HOST)
+
+
+def replication_factors(t, cutoff: float, twice: bool = False):
+    """julia_codes/framework.jl:162-211 restated: per axis start at 1; while the
+    distance from the supercell centre (a+b+c)/2 to the plane through the
+    origin spanned by the other two original cell vectors is below the cutoff
+    (doubled for "twice"), add 1 and double that cell vector."""
+    t = np.asarray(t, dtype=np.float64).reshape(3, 3)
+    rc = 2.0 * cutoff if twice else cutoff
+    v = [t[:, 0].copy(), t[:, 1].copy(), t[:, 2].copy()]
+    normals = [np.cross(v[1], v[2]), np.cross(v[0], v[2]), np.cross(v[0], v[1])]
+    rep = [1, 1, 1]
+    for k in range(3):
+        nrm = normals[k]
+        while abs(float(np.dot(nrm, 0.5 * (v[0] + v[1] + v[2])))) / float(np.linalg.norm(nrm)) < rc:
+            rep[k] += 1
+            v[k] = 2.0 * v[k]
+    return rep

@@ -94,3 +94,19 @@ def test_fp64_grid_is_refused(ctx):
         api.pockets_label(ctx, g, T)
     g.close()
     fw.close()
+
+
+def test_batch_matches_oracle_per_grid(ctx):
+    """Many grids of different shapes labelled in one call: every grid's pocket
+    count and blocked energies equal the oracle's on that grid alone."""
+    shapes = [((20, 18, 16), 0.25), ((9, 9, 9), 0.6), ((33, 35, 31), 0.29), ((17, 23, 19), 0.4),
+              ((40, 12, 30), 0.33), ((8, 8, 8), 0.0), ((2, 3, 2), 1.0)]
+    grids32 = [random_grid(100 + i, n, fill) for i, (n, fill) in enumerate(shapes)]
+    print(f"pockets batch of {len(grids32)} grids", flush=True)
+    handles = [api.Grid.upload(ctx, g) for g in grids32]
+    counts = api.pockets_label_batch(ctx, handles, T, block=True)
+    for g32, h, n in zip(grids32, handles, counts):
+        ref_n, ref_blocked, _ = oracle.find_and_block_pockets(g32.astype(np.float64), T)
+        assert n == ref_n
+        assert np.array_equal(h.download(), ref_blocked.astype(np.float32))
+        h.close()
