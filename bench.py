@@ -206,7 +206,7 @@ def bench_pockets(args, api, synth):
     voxels = sum(int(np.prod(f.grid_n())) for f in fws)
     log(f"{len(fws)} grids, {voxels:.3g} voxels in {t_grid:.2f} s")
     t0 = time.perf_counter()
-    pockets = [api.pockets_label(ctx, g, T, block=True)[0] for g in grids]
+    pockets = api.pockets_label_batch(ctx, grids, T, block=True)
     t_lab = time.perf_counter() - t0
     ins = 0
     kh = []
@@ -238,7 +238,8 @@ def bench_pockets(args, api, synth):
         "pockets_total": int(sum(pockets)), "grid_build_s": t_grid,
         "roofline": {"bound": "hbm", "achieved": lab_gbs, "peak": 8000.0, "unit": "GB/s",
                      "frac": lab_gbs / 8000.0, "traffic": None,
-                     "kernel": "pocket labelling (whole call, 8 B/voxel)"},
+                     "kernel": "pocket labelling of all grids (one er_pockets_label_batch "
+                               "call, 8 B/voxel)"},
         "trilinear_GBps_at_32B": tri_gbs,
     }
 

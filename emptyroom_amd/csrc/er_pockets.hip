@@ -3,11 +3,12 @@
 //
 // Segments: voxels with E < 15 T, 6-connected, no periodic images.  Any
 // number of grids run together, one launch per phase, blocks mapped to grids
-// through a table.  A lock-free union-find links every accessible voxel to
-// its accessible -x, -y and -z neighbours, always hooking the larger root
-// under the smaller, so the final root of a segment is its smallest linear
-// index k + j*nz + i*ny*nz -- the voxel where the reference's raster-order
-// flood fill starts it.  The segment id is the rank of that root among all
+// through a table.  Each 8x8x8 brick is first labelled by a union-find in
+// LDS; a lock-free union-find in global memory then joins accessible voxels
+// across brick faces.  Both always hook the larger root under the smaller,
+// so the final root of a segment is its smallest linear index
+// k + j*nz + i*ny*nz -- the voxel where the reference's raster-order flood
+// fill starts it.  The segment id is the rank of that root among all
 // roots; roots are kept as a bitmask (one ballot per 64 voxels) with a
 // prefix count per 64-bit word, so rank(r) = prefix[r/64] +
 // popcount(mask[r/64] below r) and no per-voxel rank array exists.  Face
@@ -90,29 +91,126 @@ __device__ __forceinline__ int seg_of(const CCGrid &G, long long v) {
     return (int)(G.wpre[w] + (uint32_t)__popcll(below));
 }
 
-__global__ __launch_bounds__(kWG) void k_cc_init(const CCGrid *__restrict__ grids, int ng,
-                                                 double thr) {
-    const CCGrid &G = grids[cc_grid_of(grids, ng, blockIdx.x)];
-    const long long base = (long long)(blockIdx.x - G.blk0) * kCCVox;
-    for (int k = threadIdx.x; k < kCCVox; k += kWG) {
-        const long long v = base + k;
-        if (v < G.nv) G.parent[v] = ((double)G.e[v] >= thr) ? -1 : (int)v;  // src/pocketblocking.cc:138
+// Brick of 8x8x8 voxels (grid indices i, j, k) labelled by one block.
+__device__ __forceinline__ int cc_brick_of(const CCGrid *__restrict__ g, int ng, long long blk) {
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g[mid].bblk0 <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__device__ __forceinline__ int lds_ld(const int *p) {
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+}
+
+__device__ __forceinline__ int lds_find(int *p, int v) {
+    int q = lds_ld(&p[v]);
+    while (q != v) {
+        const int g = lds_ld(&p[q]);
+        if (g != q)  // halving; g is an ancestor of v
+            __hip_atomic_store(&p[v], g, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        v = q;
+        q = g;
+    }
+    return v;
+}
+
+__device__ __forceinline__ void lds_unite(int *p, int a, int b) {
+    int ra = lds_find(p, a), rb = lds_find(p, b);
+    while (ra != rb) {
+        if (ra < rb) {
+            const int t = ra;
+            ra = rb;
+            rb = t;
+        }
+        const int old = atomicCAS(&p[ra], ra, rb);
+        if (old == ra) return;
+        ra = lds_find(p, old);
+        rb = lds_find(p, rb);
     }
 }
 
-__global__ __launch_bounds__(kWG) void k_cc_hook(const CCGrid *__restrict__ grids, int ng) {
-    const CCGrid &G = grids[cc_grid_of(grids, ng, blockIdx.x)];
-    const int nz = G.n[2], ny = G.n[1];
+// Accessibility (src/pocketblocking.cc:138) and union-find inside one brick
+// in LDS.  Local index (ii*8 + jj)*8 + kk orders a brick's voxels as their
+// grid indices do, so a local root is the smallest grid index of its local
+// component; parent[v] = that grid index (-1 if inaccessible).
+__global__ __launch_bounds__(kWG) void k_cc_local(const CCGrid *__restrict__ grids, int ng,
+                                                  double thr) {
+    __shared__ int s_p[kBrick];
+    const CCGrid &G = grids[cc_brick_of(grids, ng, blockIdx.x)];
+    const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
+    const int nbz = (nz + 7) >> 3, nby = (ny + 7) >> 3;
+    const long long b = blockIdx.x - G.bblk0;
+    const int bk = (int)(b % nbz), bj = (int)((b / nbz) % nby), bi = (int)(b / ((long long)nbz * nby));
+    const long long syz = (long long)ny * nz;
+    long long vg[kBrick / kWG];
+    bool acc[kBrick / kWG];
+#pragma unroll
+    for (int h = 0; h < kBrick / kWG; ++h) {
+        const int l = threadIdx.x + h * kWG;
+        const int i = bi * 8 + (l >> 6), j = bj * 8 + ((l >> 3) & 7), k = bk * 8 + (l & 7);
+        const bool in = i < nx && j < ny && k < nz;
+        vg[h] = in ? (long long)k + (long long)j * nz + (long long)i * syz : -1;
+        acc[h] = in && (double)G.e[vg[h]] < thr;
+        s_p[l] = acc[h] ? l : -1;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int h = 0; h < kBrick / kWG; ++h) {
+        const int l = threadIdx.x + h * kWG;
+        if (!acc[h]) continue;
+        if ((l & 7) > 0 && lds_ld(&s_p[l - 1]) >= 0) lds_unite(s_p, l, l - 1);
+        if (((l >> 3) & 7) > 0 && lds_ld(&s_p[l - 8]) >= 0) lds_unite(s_p, l, l - 8);
+        if ((l >> 6) > 0 && lds_ld(&s_p[l - 64]) >= 0) lds_unite(s_p, l, l - 64);
+    }
+    __syncthreads();
+    int r[kBrick / kWG];
+#pragma unroll
+    for (int h = 0; h < kBrick / kWG; ++h) {
+        const int l = threadIdx.x + h * kWG;
+        int q = acc[h] ? l : -1;
+        if (q >= 0)
+            while (s_p[q] != q) q = s_p[q];
+        r[h] = q;
+    }
+#pragma unroll
+    for (int h = 0; h < kBrick / kWG; ++h) {
+        if (vg[h] < 0) continue;
+        int out = -1;
+        if (r[h] >= 0) {
+            const int rl = r[h];
+            const int i = bi * 8 + (rl >> 6), j = bj * 8 + ((rl >> 3) & 7), k = bk * 8 + (rl & 7);
+            out = (int)((long long)k + (long long)j * nz + (long long)i * syz);
+        }
+        G.parent[vg[h]] = out;
+    }
+}
+
+// Unions across brick faces: every accessible voxel on a brick's -x, -y or
+// -z face with its accessible neighbour in the adjacent brick.
+__global__ __launch_bounds__(kWG) void k_cc_merge(const CCGrid *__restrict__ grids, int ng) {
+    const CCGrid &G = grids[cc_brick_of(grids, ng, blockIdx.x)];
+    const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
+    const int nbz = (nz + 7) >> 3, nby = (ny + 7) >> 3;
+    const long long b = blockIdx.x - G.bblk0;
+    const int bk = (int)(b % nbz), bj = (int)((b / nbz) % nby), bi = (int)(b / ((long long)nbz * nby));
     const long long syz = (long long)ny * nz;
     int *parent = G.parent;
-    const long long base = (long long)(blockIdx.x - G.blk0) * kCCVox;
-    for (int k = threadIdx.x; k < kCCVox; k += kWG) {
-        const long long v = base + k;
-        if (v >= G.nv || parent[v] < 0) continue;
-        const int kz = (int)(v % nz), j = (int)((v / nz) % ny), i = (int)(v / syz);
-        if (kz > 0 && parent[v - 1] >= 0) unite(parent, (int)v, (int)(v - 1));
-        if (j > 0 && parent[v - nz] >= 0) unite(parent, (int)v, (int)(v - nz));
-        if (i > 0 && parent[v - syz] >= 0) unite(parent, (int)v, (int)(v - syz));
+#pragma unroll
+    for (int h = 0; h < kBrick / kWG; ++h) {
+        const int l = threadIdx.x + h * kWG;
+        const int ii = l >> 6, jj = (l >> 3) & 7, kk = l & 7;
+        if (ii != 0 && jj != 0 && kk != 0) continue;
+        const int i = bi * 8 + ii, j = bj * 8 + jj, k = bk * 8 + kk;
+        if (i >= nx || j >= ny || k >= nz) continue;
+        const long long v = (long long)k + (long long)j * nz + (long long)i * syz;
+        if (ld(&parent[v]) < 0) continue;
+        if (kk == 0 && k > 0 && ld(&parent[v - 1]) >= 0) unite(parent, (int)v, (int)(v - 1));
+        if (jj == 0 && j > 0 && ld(&parent[v - nz]) >= 0) unite(parent, (int)v, (int)(v - nz));
+        if (ii == 0 && i > 0 && ld(&parent[v - syz]) >= 0) unite(parent, (int)v, (int)(v - syz));
     }
 }
 
@@ -346,14 +444,20 @@ int blocks_for(long long n) {
 
 long long cc_blocks(long long nv) { return (nv + kCCVox - 1) / kCCVox; }
 
-hipError_t launch_cc_label(const CCGrid *grids, int ng, long long total_blocks, double thr,
-                           int *packed, unsigned int *packed_total, unsigned int *meta,
-                           hipStream_t s) {
+long long cc_bricks(const int n[3]) {
+    return (long long)((n[0] + 7) / 8) * ((n[1] + 7) / 8) * ((n[2] + 7) / 8);
+}
+
+hipError_t launch_cc_label(const CCGrid *grids, int ng, long long total_blocks,
+                           long long total_bricks, double thr, int *packed,
+                           unsigned int *packed_total, unsigned int *meta, hipStream_t s) {
     if (ng <= 0) return hipSuccess;
-    if (total_blocks <= 0 || total_blocks > (1LL << 31) - 1) return hipErrorInvalidValue;
+    if (total_blocks <= 0 || total_blocks > (1LL << 31) - 1 || total_bricks <= 0 ||
+        total_bricks > (1LL << 31) - 1)
+        return hipErrorInvalidValue;
     const dim3 nb((unsigned)total_blocks);
-    hipLaunchKernelGGL(k_cc_init, nb, dim3(kWG), 0, s, grids, ng, thr);
-    hipLaunchKernelGGL(k_cc_hook, nb, dim3(kWG), 0, s, grids, ng);
+    hipLaunchKernelGGL(k_cc_local, dim3((unsigned)total_bricks), dim3(kWG), 0, s, grids, ng, thr);
+    hipLaunchKernelGGL(k_cc_merge, dim3((unsigned)total_bricks), dim3(kWG), 0, s, grids, ng);
     hipLaunchKernelGGL(k_cc_flatten, nb, dim3(kWG), 0, s, grids, ng);
     hipLaunchKernelGGL(k_cc_word_scan, dim3(ng), dim3(kWG), 0, s, grids);
     hipLaunchKernelGGL(k_cc_faces, dim3(ng), dim3(kWG), 0, s, grids, packed, packed_total, meta);

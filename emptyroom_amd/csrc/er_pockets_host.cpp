@@ -170,7 +170,7 @@ int label_chunk(Context *ctx, GridDev *const *grids, int ng, double temperature,
     Scratch sc{ctx, {}};
     std::vector<CCGrid> tab(ng);
     std::vector<long long> faces(ng);
-    long long nv_sum = 0, nw_sum = 0, face_sum = 0, blocks = 0;
+    long long nv_sum = 0, nw_sum = 0, face_sum = 0, blocks = 0, bricks = 0;
     for (int g = 0; g < ng; g++) {
         const int *n = grids[g]->n;
         CCGrid &c = tab[g];
@@ -180,7 +180,9 @@ int label_chunk(Context *ctx, GridDev *const *grids, int ng, double temperature,
         c.nv = (long long)n[0] * n[1] * n[2];
         c.nw = (c.nv + 63) / 64;
         c.blk0 = blocks;
+        c.bblk0 = bricks;
         c.seg0 = nv_sum;
+        bricks += cc_bricks(n);
         faces[g] = (long long)n[1] * n[2] + (long long)n[0] * n[2] + (long long)n[0] * n[1];
         blocks += cc_blocks(c.nv);
         nv_sum += c.nv;
@@ -209,7 +211,7 @@ int label_chunk(Context *ctx, GridDev *const *grids, int ng, double temperature,
     HIP_TRY(hipMemcpyAsync(d_tab, tab.data(), sizeof(CCGrid) * ng, hipMemcpyHostToDevice, s));
     HIP_TRY(launch_fill_u64(total, 0ull, 1, s));
     // src/pocketblocking.cc:130: E >= 15 T is inaccessible
-    HIP_TRY(launch_cc_label(d_tab, ng, blocks, 15.0 * temperature, packed,
+    HIP_TRY(launch_cc_label(d_tab, ng, blocks, bricks, 15.0 * temperature, packed,
                             (unsigned int *)total, meta, s));
     std::vector<unsigned int> hmeta(4 * (size_t)ng);
     HIP_TRY(hipMemcpyAsync(hmeta.data(), meta, sizeof(unsigned int) * 4 * ng,
