@@ -109,7 +109,8 @@ EXPORTS = [
     "er_server_release", "er_server_stats", "er_server_destroy", "er_host_setup",
     "er_host_run_info", "er_host_run_particles", "er_host_run_label",
     "er_host_run_write_settings", "er_host_run_release", "er_write_grid_file",
-    "er_replication_factors", "er_write_uc_file",
+    "er_replication_factors", "er_write_uc_file", "er_accessibility_grid",
+    "er_grid_file_info", "er_read_grid_file", "er_write_grid_binary", "er_grid_load",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -181,6 +182,13 @@ _SIGS = {
     "er_write_grid_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, _P, _P,
                                           ctypes.POINTER(ctypes.c_int),
                                           ctypes.POINTER(ctypes.c_double), ctypes.c_int]),
+    "er_grid_file_info": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int)]),
+    "er_read_grid_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.POINTER(ctypes.c_int), _P]),
+    "er_write_grid_binary": (ctypes.c_int, [ctypes.c_char_p, _P, _P,
+                                            ctypes.POINTER(ctypes.c_int), _P]),
+    "er_grid_load": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(_P)]),
+    "er_accessibility_grid": (ctypes.c_int, [_P, _P, ctypes.c_int, _P,
+                                             ctypes.POINTER(ctypes.c_int), _P]),
     "er_replication_factors": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_int)]),
     "er_write_uc_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double,
@@ -411,6 +419,13 @@ class Grid:
         _check(lib().er_grid_compute(ctx.h, fw.h, _iarr(n), precision, None, ctypes.byref(h)))
         return cls(h)
 
+    @classmethod
+    def load(cls, ctx: Context, path: str) -> "Grid":
+        """A text or binary grid file as a resident FP32 grid."""
+        h = _P()
+        _check(lib().er_grid_load(ctx.h, path.encode(), ctypes.byref(h)))
+        return cls(h)
+
     def download(self) -> np.ndarray:
         n = (ctypes.c_int * 3)()
         prec = ctypes.c_int()
@@ -438,6 +453,20 @@ def pockets_label(ctx: Context, grid: Grid, temperature: float, block: bool = Tr
                                   ctypes.byref(npk),
                                   seg.ctypes.data_as(ctypes.c_void_p) if segments else None))
     return npk.value, seg
+
+
+def accessibility_grid(ctx: Context, frac: np.ndarray, eps: np.ndarray, sig2: np.ndarray,
+                       t_matrix: np.ndarray, n: Sequence[int]) -> np.ndarray:
+    """Hard-sphere accessibility grid (1 accessible / 0 overlap) as float64."""
+    arr, na = particles_array(np.asarray(frac, dtype=np.float64),
+                              np.asarray(eps, dtype=np.float64),
+                              np.asarray(sig2, dtype=np.float64))
+    t = np.ascontiguousarray(np.asarray(t_matrix, dtype=np.float64).reshape(9))
+    out = np.empty(int(n[0]) * int(n[1]) * int(n[2]), dtype=np.float64)
+    _check(lib().er_accessibility_grid(ctx.h, ctypes.cast(arr, ctypes.c_void_p), na,
+                                       t.ctypes.data_as(ctypes.c_void_p), _iarr(n),
+                                       out.ctypes.data_as(ctypes.c_void_p)))
+    return out.reshape(int(n[0]), int(n[1]), int(n[2]))
 
 
 def pockets_label_batch(ctx: Context, grids: Sequence[Grid], temperature: float,
@@ -559,6 +588,26 @@ class HostRun:
         if self.h:
             lib().er_host_run_release(self.h)
             self.h = None
+
+
+def read_grid_file(path: str) -> np.ndarray:
+    """Text (reference format) or binary grid file as an FP64 array (N_x, N_y, N_z)."""
+    n = (ctypes.c_int * 3)()
+    _check(lib().er_grid_file_info(path.encode(), n))
+    out = np.empty(n[0] * n[1] * n[2], dtype=np.float64)
+    _check(lib().er_read_grid_file(path.encode(), n, out.ctypes.data_as(ctypes.c_void_p)))
+    return out.reshape(n[0], n[1], n[2])
+
+
+def write_grid_binary(path: str, grid: np.ndarray, t_matrix: np.ndarray) -> None:
+    g = np.ascontiguousarray(grid)
+    if g.dtype not in (np.float32, np.float64):
+        g = g.astype(np.float64)
+    t = np.ascontiguousarray(np.asarray(t_matrix, dtype=np.float64).reshape(9))
+    p = g.ctypes.data_as(ctypes.c_void_p)
+    _check(lib().er_write_grid_binary(path.encode(), p if g.dtype == np.float64 else None,
+                                      p if g.dtype == np.float32 else None, _iarr(g.shape),
+                                      t.ctypes.data_as(ctypes.c_void_p)))
 
 
 def write_grid_file(path: str, fmt: str, grid: np.ndarray, t_matrix: np.ndarray,

@@ -4,6 +4,8 @@
 #include <cstring>
 #include <memory>
 #include <new>
+#include <string>
+#include <vector>
 
 #include "../../include/emptyroom.h"
 #include "er_host.h"
@@ -556,6 +558,74 @@ int er_write_grid_file(const char *path, const char *format, const double *e64, 
     std::fclose(f);
     if (st != ST_OK) er::set_error("er_write_grid_file: write failed");
     return st;
+}
+
+int er_accessibility_grid(er_context *ctx, const er_particle *atoms, int n_atoms,
+                          const double t_matrix[9], const int n[3], double *out) {
+    if (!ctx || !t_matrix || !n || !out || n_atoms < 0 || (n_atoms > 0 && !atoms))
+        return arg_error("er_accessibility_grid: bad arguments");
+    return guarded([&]() -> int {
+        return er::accessibility_grid(C(ctx), reinterpret_cast<const er::Particle *>(atoms),
+                                      n_atoms, t_matrix, n, out);
+    });
+}
+
+int er_grid_file_info(const char *path, int n[3]) {
+    if (!path || !n) return arg_error("er_grid_file_info: null argument");
+    return guarded([&]() -> int {
+        std::string msg;
+        std::vector<double> v;
+        int st = er::host::read_grid_file(path, n, &v, 16, &msg);
+        if (st) er::set_error(msg);
+        return st;
+    });
+}
+
+int er_read_grid_file(const char *path, const int n_expected[3], double *out) {
+    if (!path || !n_expected || !out) return arg_error("er_read_grid_file: null argument");
+    return guarded([&]() -> int {
+        std::string msg;
+        std::vector<double> v;
+        int n[3];
+        int st = er::host::read_grid_file(path, n, &v, 16, &msg);
+        if (st) {
+            er::set_error(msg);
+            return st;
+        }
+        if (n[0] != n_expected[0] || n[1] != n_expected[1] || n[2] != n_expected[2])
+            return arg_error("er_read_grid_file: grid dimensions differ from n_expected");
+        std::memcpy(out, v.data(), sizeof(double) * v.size());
+        return ST_OK;
+    });
+}
+
+int er_write_grid_binary(const char *path, const double *e64, const float *e32, const int n[3],
+                         const double t_matrix[9]) {
+    if (!path || !n || !t_matrix || (!e64 == !e32))
+        return arg_error("er_write_grid_binary: bad arguments");
+    std::string msg;
+    int st = er::host::write_grid_bin(path, n, t_matrix, e64, e32, &msg);
+    if (st) er::set_error(msg);
+    return st;
+}
+
+int er_grid_load(er_context *ctx, const char *path, er_grid **out) {
+    if (!ctx || !path || !out) return arg_error("er_grid_load: null argument");
+    return guarded([&]() -> int {
+        std::string msg;
+        std::vector<double> v;
+        int n[3];
+        int st = er::host::read_grid_file(path, n, &v, 16, &msg);
+        if (st) {
+            er::set_error(msg);
+            return st;
+        }
+        std::vector<float> f(v.begin(), v.end());
+        er::GridDev *g = nullptr;
+        st = er::grid_upload(C(ctx), f.data(), n, &g);
+        if (st == ST_OK) *out = reinterpret_cast<er_grid *>(g);
+        return st;
+    });
 }
 
 int er_replication_factors(const double t_matrix[9], double cutoff, int twice, int rep[3]) {

@@ -5,8 +5,10 @@
 #include "er_host.h"
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstdarg>
+#include <cstdlib>
 #include <cstring>
 #include <ctime>
 #include <fstream>
@@ -91,6 +93,7 @@ int read_grid_settings(const std::string &root, GridParams *p, std::string *msg)
         if (word == "GridOutputFormat") simfile >> p->gridoutputformat;
         if (word == "FeynmanHibbs") simfile >> p->feynmanhibbs;
         if (word == "verbosemode") simfile >> p->verbose;
+        if (word == "AccessibilityGrid") simfile >> p->accessibility;  // extension
         if (word == "GridPrecision") {  // extension, ignored by the reference scanner
             std::string v;
             simfile >> v;
@@ -359,6 +362,201 @@ long long henry_total(const HenryParams &p) {
     return (long long)p.numinsertionsperthread * p.num_blocks * p.num_threads;
 }
 
+namespace {
+
+struct BinHeader {
+    char magic[8];
+    uint32_t version, dtype;
+    int32_t n[3];
+    uint32_t reserved;
+    double t[9];
+    uint8_t pad[128 - 8 - 8 - 16 - 72];
+};
+static_assert(sizeof(BinHeader) == 128, "binary grid header is 128 bytes");
+constexpr char kBinMagic[8] = {'E', 'R', 'G', 'R', 'I', 'D', '0', '1'};
+
+bool slurp(const std::string &path, std::string *data) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) return false;
+    std::fseek(f, 0, SEEK_END);
+    const long sz = std::ftell(f);
+    std::fseek(f, 0, SEEK_SET);
+    data->resize(sz > 0 ? (size_t)sz : 0);
+    const size_t got = sz > 0 ? fread(&(*data)[0], 1, (size_t)sz, f) : 0;
+    fclose(f);
+    return got == data->size();
+}
+
+}  // namespace
+
+int read_grid_txt(const std::string &path, int n[3], std::vector<double> *vals, int threads,
+                  std::string *msg) {
+    std::string data;
+    if (!slurp(path, &data)) {
+        *msg = fmt("Grid file %s could not be loaded...\n", path.c_str());  // src/gcmc.cc:297
+        return ST_IO;
+    }
+    const size_t eol = data.find('\n');
+    {
+        std::istringstream head(data.substr(0, eol));
+        head >> n[0] >> n[1] >> n[2];
+        if (head.fail() || n[0] < 1 || n[1] < 1 || n[2] < 1 ||
+            (long long)n[0] * n[1] * n[2] > (1LL << 36)) {
+            *msg = fmt("Grid file %s has no valid N_x N_y N_z header", path.c_str());
+            return ST_PARSE;
+        }
+    }
+    const long long nlines = (long long)n[0] * n[1];
+    std::vector<size_t> start;
+    start.reserve((size_t)nlines + 1);
+    size_t pos = eol == std::string::npos ? data.size() : eol + 1;
+    while (pos < data.size() && (long long)start.size() < nlines) {
+        start.push_back(pos);
+        const void *nl = std::memchr(data.data() + pos, '\n', data.size() - pos);
+        pos = nl ? (size_t)((const char *)nl - data.data()) + 1 : data.size();
+    }
+    if ((long long)start.size() < nlines) {
+        *msg = fmt("Grid file %s has %lld of %lld z-pencil lines", path.c_str(),
+                   (long long)start.size(), nlines);
+        return ST_PARSE;
+    }
+    start.push_back(pos);
+    vals->assign((size_t)nlines * n[2], 0.0);
+    std::atomic<long long> bad{-1};
+    auto work = [&](long long l0, long long l1) {
+        for (long long l = l0; l < l1 && bad.load() < 0; l++) {
+            const char *p = data.data() + start[l];
+            const char *end = data.data() + start[l + 1];
+            double *out = vals->data() + (size_t)l * n[2];
+            for (int k = 0; k < n[2]; k++) {
+                while (p < end && (*p == ' ' || *p == '\t' || *p == '\r')) p++;
+                char *q = nullptr;
+                const double v = std::strtod(p, &q);
+                if (q == p || q > end || !std::isfinite(v)) {
+                    long long expect = -1;
+                    bad.compare_exchange_strong(expect, l);
+                    return;
+                }
+                out[k] = v;
+                p = q;
+            }
+        }
+    };
+    const int nt = std::max(1, std::min<int>(threads, (int)std::min<long long>(nlines, 64)));
+    std::vector<std::thread> pool;
+    for (int t = 0; t < nt; t++)
+        pool.emplace_back(work, nlines * t / nt, nlines * (t + 1) / nt);
+    for (auto &th : pool) th.join();
+    if (bad.load() >= 0) {
+        *msg = fmt("Grid file %s: line %lld does not hold %d values", path.c_str(),
+                   bad.load() + 2, n[2]);
+        return ST_PARSE;
+    }
+    return ST_OK;
+}
+
+int write_grid_bin(const std::string &path, const int n[3], const double t[9], const double *e64,
+                   const float *e32, std::string *msg) {
+    BinHeader h;
+    std::memset(&h, 0, sizeof(h));
+    std::memcpy(h.magic, kBinMagic, 8);
+    h.version = 1;
+    h.dtype = e64 ? 1u : 0u;
+    for (int d = 0; d < 3; d++) h.n[d] = n[d];
+    std::memcpy(h.t, t, sizeof(h.t));
+    FILE *f = fopen(path.c_str(), "wb");
+    if (!f) {
+        *msg = fmt("could not write %s", path.c_str());
+        return ST_IO;
+    }
+    const size_t np = (size_t)n[0] * n[1] * n[2];
+    bool ok = fwrite(&h, sizeof(h), 1, f) == 1;
+    ok = ok && (e64 ? fwrite(e64, sizeof(double), np, f) : fwrite(e32, sizeof(float), np, f)) == np;
+    ok = (fclose(f) == 0) && ok;
+    if (!ok) {
+        *msg = fmt("could not write %s", path.c_str());
+        return ST_IO;
+    }
+    return ST_OK;
+}
+
+int read_grid_bin(const std::string &path, int n[3], double t[9], int *dtype,
+                  std::vector<char> *values, std::string *msg) {
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) {
+        *msg = fmt("Grid file %s could not be loaded...\n", path.c_str());
+        return ST_IO;
+    }
+    BinHeader h;
+    const bool got = fread(&h, sizeof(h), 1, f) == 1;
+    if (!got || std::memcmp(h.magic, kBinMagic, 8) != 0 || h.version != 1 || h.dtype > 1 ||
+        h.n[0] < 1 || h.n[1] < 1 || h.n[2] < 1 ||
+        (long long)h.n[0] * h.n[1] * h.n[2] > (1LL << 36)) {
+        fclose(f);
+        *msg = fmt("%s is not a binary grid file", path.c_str());
+        return ST_PARSE;
+    }
+    for (int d = 0; d < 3; d++) n[d] = h.n[d];
+    std::memcpy(t, h.t, sizeof(h.t));
+    *dtype = (int)h.dtype;
+    const size_t bytes = (size_t)n[0] * n[1] * n[2] * (h.dtype ? 8 : 4);
+    values->resize(bytes);
+    const bool ok = fread(values->data(), 1, bytes, f) == bytes;
+    fclose(f);
+    if (!ok) {
+        *msg = fmt("binary grid file %s is truncated", path.c_str());
+        return ST_PARSE;
+    }
+    return ST_OK;
+}
+
+int read_grid_file(const std::string &path, int n[3], std::vector<double> *vals, int threads,
+                   std::string *msg) {
+    char magic[8] = {0};
+    FILE *f = fopen(path.c_str(), "rb");
+    if (!f) {
+        *msg = fmt("Grid file %s could not be loaded...\n", path.c_str());
+        return ST_IO;
+    }
+    const size_t got = fread(magic, 1, 8, f);
+    fclose(f);
+    if (got == 8 && std::memcmp(magic, kBinMagic, 8) == 0) {
+        double t[9];
+        int dtype = 0;
+        std::vector<char> raw;
+        int st = read_grid_bin(path, n, t, &dtype, &raw, msg);
+        if (st) return st;
+        const size_t np = (size_t)n[0] * n[1] * n[2];
+        vals->resize(np);
+        if (dtype) std::memcpy(vals->data(), raw.data(), np * sizeof(double));
+        else
+            for (size_t i = 0; i < np; i++) {
+                float v;
+                std::memcpy(&v, raw.data() + 4 * i, 4);
+                (*vals)[i] = v;
+            }
+        return ST_OK;
+    }
+    return read_grid_txt(path, n, vals, threads, msg);
+}
+
+int write_accessibility_values(FILE *f, const double *acc, const int n[3]) {
+    const long long syz = (long long)n[1] * n[2];
+    for (int i = 0; i < n[0]; i++)
+        for (int j = 0; j < n[1]; j++) {
+            int count = 0;
+            for (int k = 0; k < n[2]; k++) {
+                fprintf(f, "%d ", int(acc[k + (long long)j * n[2] + (long long)i * syz]));
+                if (++count == 6) {
+                    fprintf(f, "\n");
+                    count = 0;
+                }
+            }
+            fprintf(f, "\n");
+        }
+    return ferror(f) ? ST_IO : ST_OK;
+}
+
 void flat_t(const Framework &fw, double t[9]) {
     for (int i = 0; i < 3; i++)
         for (int j = 0; j < 3; j++) t[3 * i + j] = fw.t_matrix[i][j];
@@ -618,6 +816,8 @@ int setup_grid_run(const std::string &root, const std::string &structure,
         return st;
     int n[3];
     grid_dims(r->fw, p->grid_resolution, n);
+    if (p->accessibility)  // src/writegrid.cu:104-108: a grid in fractional space
+        for (int d = 0; d < 3; d++) n[d] = static_cast<int>(std::ceil(1.0 / p->grid_resolution));
     p->N_x = n[0];
     p->N_y = n[1];
     p->N_z = n[2];

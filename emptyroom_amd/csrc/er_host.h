@@ -52,6 +52,9 @@ struct GridParams {
     int N_x = 0, N_y = 0, N_z = 0;
     int rep[3] = {0, 0, 0};
     int precision = 1;  // 1 FP64 (reference arithmetic), 0 FP32 grid
+    // Extension key AccessibilityGrid 0|1: the hard-sphere branch of
+    // src/writegrid.cu:57-59,104-108,272-298, which the reference hardcodes off.
+    bool accessibility = false;
 };
 
 // Settings of a henry run (src/datatypes.h:44-76).  `seed` is an extension
@@ -109,6 +112,29 @@ void write_txt_header(FILE *f, const int n[3]);
 void write_cube_header(FILE *f, const int n[3], const double t[3][3]);
 int write_grid_values(FILE *f, const std::string &format, const double *e64, const float *e32,
                       const int n[3], int threads);
+
+// Text grid (src/writegrid.cu:144-147,260-269) read as src/gcmc.cc:286-346
+// does: a header line "N_x N_y N_z ...", then N_x*N_y lines, each the N_z
+// values of one z-pencil (j fastest, then i); layout k + j*nz + i*ny*nz.
+// Values beyond N_z on a line are ignored, as the reference's istream loop
+// does; a short line, a missing line or a value that does not parse is an
+// error (the reference would keep zeros or fail an assert).
+int read_grid_txt(const std::string &path, int n[3], std::vector<double> *vals, int threads,
+                  std::string *msg);
+// Binary side-format: a 128-byte header {"ERGRID01", version 1, dtype
+// (0 FP32, 1 FP64), N_x, N_y, N_z, 0, t_matrix[9] row-major, zero padding},
+// then the values in the layout above, little-endian.
+int write_grid_bin(const std::string &path, const int n[3], const double t[9], const double *e64,
+                   const float *e32, std::string *msg);
+int read_grid_bin(const std::string &path, int n[3], double t[9], int *dtype,
+                  std::vector<char> *values, std::string *msg);
+// Either format, by content (binary magic, else text), as FP64 values.
+int read_grid_file(const std::string &path, int n[3], std::vector<double> *vals, int threads,
+                   std::string *msg);
+
+// Accessibility values of src/writegrid.cu:285-297: "%d " per point, a
+// newline after every 6 values and after every z-pencil.
+int write_accessibility_values(FILE *f, const double *acc, const int n[3]);
 
 // Full host-side setup of a run, in the order of src/writegrid.cu:64-98 and
 // src/henry.cu:50-92 (stops at the first error, like the reference).

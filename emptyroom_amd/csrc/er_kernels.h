@@ -11,7 +11,7 @@
 namespace er {
 
 // Pair-math variants of the tiled energy kernel.
-enum Math : int { MATH_MIXED = 0, MATH_FP64 = 1, MATH_COUNT = 2 };
+enum Math : int { MATH_MIXED = 0, MATH_FP64 = 1, MATH_COUNT = 2, MATH_OVERLAP = 3 };
 
 // Framework atom in fractional coordinates with its type index (atoms are
 // grouped by type before upload).

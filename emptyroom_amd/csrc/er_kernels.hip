@@ -309,7 +309,7 @@ __device__ __forceinline__ Tile tile_sphere(const double *t, const double lo[3],
 template <int MATH>
 __device__ __forceinline__ int near_region_start(int cap) {
     if (MATH == MATH_FP64) return kWG;        // every record is near
-    if (MATH == MATH_COUNT) return cap - kWG; // no near records
+    if (MATH == MATH_COUNT || MATH == MATH_OVERLAP) return cap - kWG;  // no near records
     // both regions hold at least one chunk of kWG records (cap >= 2 kWG), so a
     // fill pass that starts empty always makes progress
     const int near_cap = (cap / 3) & ~63;
@@ -380,7 +380,7 @@ __device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict
             if (d2 < in2) {
                 if (MATH == MATH_FP64) cat = CAT_NEAR;
                 else if (MATH == MATH_MIXED && d2 <= near2) cat = CAT_NEAR;
-                else cat = d2 < far2 ? CAT_FAR : CAT_SHELL;
+                else cat = (MATH != MATH_OVERLAP && d2 < far2) ? CAT_FAR : CAT_SHELL;
             }
         }
         const uint64_t bf = __ballot(cat == CAT_FAR);
@@ -460,6 +460,20 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
         const int sb = uniform(seg[kSegInts * s + 3]), se = uniform(seg[kSegInts * s + 4]);
         const int nb = uniform(seg[kSegInts * s + 5]), ne = uniform(seg[kSegInts * s + 6]);
         const TypeParams P = J->types[t];
+        if (MATH == MATH_OVERLAP) {
+            // src/computegridsheet.h:154: overlap iff r^2 < sigma_ij^2 (every
+            // record is a shell record: the cutoff is sigma_max)
+            const double sig2 = P.sig2;
+            for (int k = sb; k < se; ++k) {
+                const double4 A = Ls[-k];
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) {
+                    const double r2 = fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
+                    acc[q] += r2 < sig2 ? 1.0 : 0.0;
+                }
+            }
+            continue;
+        }
         if (MATH == MATH_COUNT) {
 #pragma unroll
             for (int q = 0; q < PPL; ++q) acc[q] += (double)(fe - fb);
@@ -614,6 +628,8 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
         if (!ok[q]) continue;
         if (MATH == MATH_COUNT) {
             ((uint32_t *)J->out)[oidx[q]] = (uint32_t)acc[q];
+        } else if (MATH == MATH_OVERLAP) {
+            ((double *)J->out)[oidx[q]] = acc[q] > 0.0 ? 0.0 : 1.0;  // src/computegridsheet.h:159-162
         } else if (J->mode == OUT_F64) {
             ((double *)J->out)[oidx[q]] = acc[q];
         } else {
@@ -976,6 +992,7 @@ hipError_t launch_tiles(int math, int ppl, const JobDev *jobs, int njobs, long l
     ER_TILES(1, MATH_MIXED) ER_TILES(2, MATH_MIXED)
     ER_TILES(1, MATH_FP64) ER_TILES(2, MATH_FP64)
     ER_TILES(1, MATH_COUNT) ER_TILES(2, MATH_COUNT)
+    ER_TILES(1, MATH_OVERLAP) ER_TILES(2, MATH_OVERLAP)
 #undef ER_TILES
     return hipErrorInvalidValue;
 }
@@ -986,6 +1003,7 @@ int tiles_occupancy(int math, int ppl, int cap) {
     ER_OCC(1, MATH_MIXED) ER_OCC(2, MATH_MIXED)
     ER_OCC(1, MATH_FP64) ER_OCC(2, MATH_FP64)
     ER_OCC(1, MATH_COUNT) ER_OCC(2, MATH_COUNT)
+    ER_OCC(1, MATH_OVERLAP) ER_OCC(2, MATH_OVERLAP)
 #undef ER_OCC
     return 1;
 }

@@ -11,6 +11,7 @@ namespace er {
 
 constexpr int kCCVox = 2048;  // voxels per block of the per-voxel labelling passes
 constexpr int kBrick = 512;   // voxels of an 8x8x8 brick (local union-find in LDS)
+constexpr int kFaceChunk = 4096;  // face pairs per block of the link pass
 
 // One FP32 grid of a labelling launch.  Plain-old-data, copied to the device.
 struct CCGrid {
@@ -26,18 +27,19 @@ struct CCGrid {
     const uint8_t *pocket;     // segments: 1 if pocket (blocking pass), or null
     long long blk0;            // first block of this grid in per-voxel launches
     long long bblk0;           // first block of this grid in per-brick launches
+    long long fblk0;           // first block of this grid in the face-link launch
     long long seg0;            // offset of this grid in a segment-id output
 };
 
 long long cc_blocks(long long nv);
 long long cc_bricks(const int n[3]);
+long long cc_face_blocks(const int n[3]);
 // Brick-local labelling, face merges, flatten, word scan and face links of
-// every grid.  packed receives each grid's links at offset meta[4g+2] (taken
-// from *packed_total, which must start at 0); meta[4g] = links,
-// meta[4g+1] = segments.
+// every grid.  packed receives grid g's links (3 ints each) at offset
+// meta[4g+2]; meta[4g] = links, meta[4g+1] = segments.
 hipError_t launch_cc_label(const CCGrid *grids, int ng, long long total_blocks,
-                           long long total_bricks, double thr, int *packed,
-                           unsigned int *packed_total, unsigned int *meta, hipStream_t s);
+                           long long total_bricks, long long total_face_blocks, double thr,
+                           int *packed, unsigned int *meta, hipStream_t s);
 // seg_out[seg0 + v] = segment id of voxel v (-1 inaccessible), after labelling.
 hipError_t launch_cc_segments(const CCGrid *grids, int ng, long long total_blocks, int *seg_out,
                               hipStream_t s);

@@ -207,10 +207,34 @@ __global__ __launch_bounds__(kWG) void k_cc_merge(const CCGrid *__restrict__ gri
         const int i = bi * 8 + ii, j = bj * 8 + jj, k = bk * 8 + kk;
         if (i >= nx || j >= ny || k >= nz) continue;
         const long long v = (long long)k + (long long)j * nz + (long long)i * syz;
-        if (ld(&parent[v]) < 0) continue;
-        if (kk == 0 && k > 0 && ld(&parent[v - 1]) >= 0) unite(parent, (int)v, (int)(v - 1));
-        if (jj == 0 && j > 0 && ld(&parent[v - nz]) >= 0) unite(parent, (int)v, (int)(v - nz));
-        if (ii == 0 && i > 0 && ld(&parent[v - syz]) >= 0) unite(parent, (int)v, (int)(v - syz));
+        const int pv = ld(&parent[v]);
+        if (pv < 0) continue;
+        // A union is skipped when the face voxel just before this one along
+        // one of the face's two axes has the same (parent, neighbour's parent)
+        // pair: both are ancestors of both pairs, so that union (or the one
+        // it in turn relies on, always at a smaller face position) joins
+        // these two sets as well.
+        if (kk == 0 && k > 0) {
+            const int pn = ld(&parent[v - 1]);
+            bool dup = false;
+            if (jj > 0) dup |= ld(&parent[v - nz]) == pv && ld(&parent[v - nz - 1]) == pn;
+            if (ii > 0) dup |= ld(&parent[v - syz]) == pv && ld(&parent[v - syz - 1]) == pn;
+            if (pn >= 0 && !dup) unite(parent, (int)v, (int)(v - 1));
+        }
+        if (jj == 0 && j > 0) {
+            const int pn = ld(&parent[v - nz]);
+            bool dup = false;
+            if (kk > 0) dup |= ld(&parent[v - 1]) == pv && ld(&parent[v - nz - 1]) == pn;
+            if (ii > 0) dup |= ld(&parent[v - syz]) == pv && ld(&parent[v - syz - nz]) == pn;
+            if (pn >= 0 && !dup) unite(parent, (int)v, (int)(v - nz));
+        }
+        if (ii == 0 && i > 0) {
+            const int pn = ld(&parent[v - syz]);
+            bool dup = false;
+            if (kk > 0) dup |= ld(&parent[v - 1]) == pv && ld(&parent[v - syz - 1]) == pn;
+            if (jj > 0) dup |= ld(&parent[v - nz]) == pv && ld(&parent[v - syz - nz]) == pn;
+            if (pn >= 0 && !dup) unite(parent, (int)v, (int)(v - syz));
+        }
     }
 }
 
@@ -278,36 +302,51 @@ __global__ __launch_bounds__(kWG) void k_cc_word_scan(const CCGrid *__restrict__
     if (tid == 0) G.wpre[G.nw] = s_carry;
 }
 
+__device__ __forceinline__ int cc_face_grid_of(const CCGrid *__restrict__ g, int ng,
+                                                long long blk) {
+    int lo = 0, hi = ng - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (g[mid].fblk0 <= blk) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+__global__ void k_cc_zero(unsigned int *__restrict__ meta, int n) {
+    for (int i = threadIdx.x; i < n; i += blockDim.x) meta[i] = 0u;
+}
+
 // Face links of src/pocketblocking.cc:273-321: for each face pair the segment
 // at index N-1 links to the segment at index 0 in direction +1/+2/+3.  Pairs
 // with an inaccessible side are dropped, and so is every pair equal to the
-// one before it in face order (repeats change no classification).  One
-// block per grid writes its compacted (from, to, dir) triples at the start of
-// its region, then copies them to the packed buffer at an offset taken from
-// *packed_total; meta[4g..4g+3] = (links, segments, packed offset, 0).
-__global__ __launch_bounds__(kWG) void k_cc_faces(const CCGrid *__restrict__ grids,
-                                                  int *__restrict__ packed,
-                                                  unsigned int *__restrict__ packed_total,
+// one before it in face order within a block's chunk (repeats change no
+// classification, nor does the order of the links).  A block takes
+// kFaceChunk face pairs of one grid, compacts them in LDS, then reserves its
+// slots in the grid's link region with one atomic on meta[4g].
+__global__ __launch_bounds__(kWG) void k_cc_faces(const CCGrid *__restrict__ grids, int ng,
                                                   unsigned int *__restrict__ meta) {
-    const CCGrid &G = grids[blockIdx.x];
+    __shared__ int s_l[3 * kFaceChunk];
     __shared__ int s_a[kWG], s_b[kWG], s_d[kWG];
     __shared__ int s_cnt[kWG / 64];
-    __shared__ int s_prev[3];
     __shared__ unsigned int s_base;
+    const int g = cc_face_grid_of(grids, ng, blockIdx.x);
+    const CCGrid &G = grids[g];
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
     const long long fx = (long long)ny * nz, fy = (long long)nx * nz, fz = (long long)nx * ny;
     const long long total = fx + fy + fz;
-    if (tid == 0) s_prev[0] = s_prev[1] = s_prev[2] = -2;
+    const long long c0 = (long long)(blockIdx.x - G.fblk0) * kFaceChunk;
+    const long long c1 = c0 + kFaceChunk < total ? c0 + kFaceChunk : total;
+    int pa = -2, pb = -2, pd = -2;  // previous pair (block-uniform)
     int cnt = 0;
-    __syncthreads();
-    for (long long q0 = 0; q0 < total; q0 += kWG) {
+    for (long long q0 = c0; q0 < c1; q0 += kWG) {
         const long long q = q0 + tid;
         int a = -1, b = -1, dir = 0;
-        if (q < total) {
+        if (q < c1) {
             long long here, there;
             if (q < fx) {
-                here = (q / nz) * nz + (q % nz);
+                here = q;
                 there = here + (long long)(nx - 1) * ny * nz;
                 dir = 1;
             } else if (q < fx + fy) {
@@ -328,12 +367,15 @@ __global__ __launch_bounds__(kWG) void k_cc_faces(const CCGrid *__restrict__ gri
         s_b[tid] = b;
         s_d[tid] = dir;
         __syncthreads();
-        const int pa = tid > 0 ? s_a[tid - 1] : s_prev[0];
-        const int pb = tid > 0 ? s_b[tid - 1] : s_prev[1];
-        const int pd = tid > 0 ? s_d[tid - 1] : s_prev[2];
-        const bool keep = q < total && a >= 0 && b >= 0 && !(a == pa && b == pb && dir == pd);
+        const int qa = tid > 0 ? s_a[tid - 1] : pa;
+        const int qb = tid > 0 ? s_b[tid - 1] : pb;
+        const int qd = tid > 0 ? s_d[tid - 1] : pd;
+        const bool keep = q < c1 && a >= 0 && b >= 0 && !(a == qa && b == qb && dir == qd);
         const unsigned long long bk = __ballot(keep);
         if (lane == 0) s_cnt[wave] = __popcll(bk);
+        pa = s_a[kWG - 1];
+        pb = s_b[kWG - 1];
+        pd = s_d[kWG - 1];
         __syncthreads();
         int off = cnt, tot = 0;
 #pragma unroll
@@ -342,32 +384,36 @@ __global__ __launch_bounds__(kWG) void k_cc_faces(const CCGrid *__restrict__ gri
             tot += s_cnt[w];
         }
         if (keep) {
-            const int o = off + __builtin_amdgcn_mbcnt_hi((uint32_t)(bk >> 32),
-                                                          __builtin_amdgcn_mbcnt_lo((uint32_t)bk, 0u));
-            G.links[3 * o] = a;
-            G.links[3 * o + 1] = b;
-            G.links[3 * o + 2] = dir;
+            const int o = off + __builtin_amdgcn_mbcnt_hi(
+                                    (uint32_t)(bk >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)bk, 0u));
+            s_l[3 * o] = a;
+            s_l[3 * o + 1] = b;
+            s_l[3 * o + 2] = dir;
         }
         cnt += tot;
         __syncthreads();
-        if (tid == kWG - 1) {
-            s_prev[0] = a;
-            s_prev[1] = b;
-            s_prev[2] = dir;
-        }
-        __syncthreads();
     }
-    // the block's own link writes are read back by every wave below
-    __threadfence();
-    if (tid == 0) s_base = atomicAdd(packed_total, (unsigned int)cnt);
+    if (tid == 0) s_base = cnt > 0 ? atomicAdd(&meta[4 * g], (unsigned int)cnt) : 0u;
     __syncthreads();
     const unsigned int base = s_base;
-    for (int k = tid; k < 3 * cnt; k += kWG) packed[3 * (long long)base + k] = G.links[k];
-    if (tid == 0) {
-        meta[4 * blockIdx.x] = (unsigned int)cnt;
-        meta[4 * blockIdx.x + 1] = G.wpre[G.nw];
-        meta[4 * blockIdx.x + 2] = base;
-        meta[4 * blockIdx.x + 3] = 0u;
+    for (int k = tid; k < 3 * cnt; k += kWG) G.links[3 * (long long)base + k] = s_l[k];
+}
+
+// One block per grid: its links go to packed[3 * off_g ...] with off_g the
+// links of all earlier grids; meta[4g+1] = segments, meta[4g+2] = off_g.
+__global__ __launch_bounds__(kWG) void k_cc_pack(const CCGrid *__restrict__ grids,
+                                                 int *__restrict__ packed,
+                                                 unsigned int *__restrict__ meta) {
+    const int g = blockIdx.x;
+    const CCGrid &G = grids[g];
+    unsigned int off = 0;
+    for (int h = 0; h < g; ++h) off += meta[4 * h];
+    const unsigned int cnt = meta[4 * g];
+    for (long long k = threadIdx.x; k < 3LL * cnt; k += kWG)
+        packed[3 * (long long)off + k] = G.links[k];
+    if (threadIdx.x == 0) {
+        meta[4 * g + 1] = G.wpre[G.nw];
+        meta[4 * g + 2] = off;
     }
 }
 
@@ -448,19 +494,28 @@ long long cc_bricks(const int n[3]) {
     return (long long)((n[0] + 7) / 8) * ((n[1] + 7) / 8) * ((n[2] + 7) / 8);
 }
 
+long long cc_face_blocks(const int n[3]) {
+    const long long f = (long long)n[1] * n[2] + (long long)n[0] * n[2] + (long long)n[0] * n[1];
+    return (f + kFaceChunk - 1) / kFaceChunk;
+}
+
 hipError_t launch_cc_label(const CCGrid *grids, int ng, long long total_blocks,
-                           long long total_bricks, double thr, int *packed,
-                           unsigned int *packed_total, unsigned int *meta, hipStream_t s) {
+                           long long total_bricks, long long total_face_blocks, double thr,
+                           int *packed, unsigned int *meta, hipStream_t s) {
     if (ng <= 0) return hipSuccess;
     if (total_blocks <= 0 || total_blocks > (1LL << 31) - 1 || total_bricks <= 0 ||
-        total_bricks > (1LL << 31) - 1)
+        total_bricks > (1LL << 31) - 1 || total_face_blocks <= 0 ||
+        total_face_blocks > (1LL << 31) - 1 || 4LL * ng > (1LL << 30))
         return hipErrorInvalidValue;
     const dim3 nb((unsigned)total_blocks);
     hipLaunchKernelGGL(k_cc_local, dim3((unsigned)total_bricks), dim3(kWG), 0, s, grids, ng, thr);
     hipLaunchKernelGGL(k_cc_merge, dim3((unsigned)total_bricks), dim3(kWG), 0, s, grids, ng);
     hipLaunchKernelGGL(k_cc_flatten, nb, dim3(kWG), 0, s, grids, ng);
     hipLaunchKernelGGL(k_cc_word_scan, dim3(ng), dim3(kWG), 0, s, grids);
-    hipLaunchKernelGGL(k_cc_faces, dim3(ng), dim3(kWG), 0, s, grids, packed, packed_total, meta);
+    hipLaunchKernelGGL(k_cc_zero, dim3(1), dim3(kWG), 0, s, meta, 4 * ng);
+    hipLaunchKernelGGL(k_cc_faces, dim3((unsigned)total_face_blocks), dim3(kWG), 0, s, grids, ng,
+                       meta);
+    hipLaunchKernelGGL(k_cc_pack, dim3(ng), dim3(kWG), 0, s, grids, packed, meta);
     return hipGetLastError();
 }
 

@@ -170,7 +170,7 @@ int label_chunk(Context *ctx, GridDev *const *grids, int ng, double temperature,
     Scratch sc{ctx, {}};
     std::vector<CCGrid> tab(ng);
     std::vector<long long> faces(ng);
-    long long nv_sum = 0, nw_sum = 0, face_sum = 0, blocks = 0, bricks = 0;
+    long long nv_sum = 0, nw_sum = 0, face_sum = 0, blocks = 0, bricks = 0, fblocks = 0;
     for (int g = 0; g < ng; g++) {
         const int *n = grids[g]->n;
         CCGrid &c = tab[g];
@@ -181,8 +181,10 @@ int label_chunk(Context *ctx, GridDev *const *grids, int ng, double temperature,
         c.nw = (c.nv + 63) / 64;
         c.blk0 = blocks;
         c.bblk0 = bricks;
+        c.fblk0 = fblocks;
         c.seg0 = nv_sum;
         bricks += cc_bricks(n);
+        fblocks += cc_face_blocks(n);
         faces[g] = (long long)n[1] * n[2] + (long long)n[0] * n[2] + (long long)n[0] * n[1];
         blocks += cc_blocks(c.nv);
         nv_sum += c.nv;
@@ -193,11 +195,9 @@ int label_chunk(Context *ctx, GridDev *const *grids, int ng, double temperature,
         *packed = sc.get<int>(3 * face_sum);
     unsigned long long *mask = sc.get<unsigned long long>(nw_sum);
     uint32_t *wpre = sc.get<uint32_t>(nw_sum);
-    unsigned long long *total = sc.get<unsigned long long>(1);
     unsigned int *meta = sc.get<unsigned int>(4 * (size_t)ng);
     CCGrid *d_tab = sc.get<CCGrid>(ng);
-    if (!parent || !links || !packed || !mask || !wpre || !total || !meta || !d_tab)
-        return ST_NOMEM;
+    if (!parent || !links || !packed || !mask || !wpre || !meta || !d_tab) return ST_NOMEM;
     long long ov = 0, ow = 0, of = 0;
     for (int g = 0; g < ng; g++) {
         tab[g].parent = parent + ov;
@@ -209,10 +209,9 @@ int label_chunk(Context *ctx, GridDev *const *grids, int ng, double temperature,
         of += faces[g];
     }
     HIP_TRY(hipMemcpyAsync(d_tab, tab.data(), sizeof(CCGrid) * ng, hipMemcpyHostToDevice, s));
-    HIP_TRY(launch_fill_u64(total, 0ull, 1, s));
     // src/pocketblocking.cc:130: E >= 15 T is inaccessible
-    HIP_TRY(launch_cc_label(d_tab, ng, blocks, bricks, 15.0 * temperature, packed,
-                            (unsigned int *)total, meta, s));
+    HIP_TRY(launch_cc_label(d_tab, ng, blocks, bricks, fblocks, 15.0 * temperature, packed, meta,
+                            s));
     std::vector<unsigned int> hmeta(4 * (size_t)ng);
     HIP_TRY(hipMemcpyAsync(hmeta.data(), meta, sizeof(unsigned int) * 4 * ng,
                            hipMemcpyDeviceToHost, s));

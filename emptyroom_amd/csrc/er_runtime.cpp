@@ -2,6 +2,7 @@
 #include "er_runtime.h"
 
 #include <algorithm>
+#include <memory>
 #include <cmath>
 #include <cstring>
 #include <sstream>
@@ -210,7 +211,7 @@ int tiles_workgroups(const Context *ctx, int math, long long items, int scratch_
 
 int scratch_ensure(Context *ctx, hipStream_t s, double4 **buf, int *wgs) {
     int occ = 1;
-    for (int m : {(int)MATH_MIXED, (int)MATH_FP64, (int)MATH_COUNT})
+    for (int m : {(int)MATH_MIXED, (int)MATH_FP64, (int)MATH_COUNT, (int)MATH_OVERLAP})
         occ = std::max(occ, tiles_occupancy(m, ctx->tuning.ppl, ctx->tuning.cap));
     int need = ctx->n_cu * occ;
     if (*buf && *wgs >= need) return ST_OK;
@@ -746,6 +747,39 @@ int count_incut_grid(Context *ctx, FrameworkDev *fw, const int n[3], unsigned lo
     HIP_TRY(launch_fill_u64(d_sum, 0ull, 1, s));
     HIP_TRY(launch_sum_u32(d_cnt, p, d_sum, s));
     HIP_TRY(hipMemcpyAsync(out, d_sum, sizeof(*out), hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ST_OK;
+}
+
+int accessibility_grid(Context *ctx, const Particle *atoms, int n_atoms, const double t[9],
+                       const int n[3], double *host_out) {
+    if (!ctx || (n_atoms > 0 && !atoms) || n_atoms < 0 || !t || !n || !host_out) {
+        set_error("accessibility_grid: bad arguments");
+        return ST_ARG;
+    }
+    ST_TRY(check_grid_dims(n));
+    ST_TRY(check_tuning(ctx->tuning));
+    // candidates: images [-1, 1]^3 within sigma_max of the cell
+    // (src/computegridsheet.h:131-133 loops those images whatever the
+    // replication factors; an atom farther than sigma_max overlaps nothing)
+    double sig2_max = 0.0;
+    for (int i = 0; i < n_atoms; i++) sig2_max = std::max(sig2_max, atoms[i].sig2);
+    const int rep1[3] = {1, 1, 1};
+    FrameworkDev *fw = nullptr;
+    ST_TRY(framework_upload(ctx, atoms, n_atoms, t, rep1, std::max(sig2_max, 1e-300), &fw));
+    std::unique_ptr<FrameworkDev> owner(fw);
+    HIP_TRY(hipSetDevice(ctx->device));
+    Scoped tmp{ctx, {}};
+    const long long p = (long long)n[0] * n[1] * n[2];
+    double *d_out = nullptr;
+    JobDev *d_job = nullptr;
+    ST_TRY(pool_alloc(ctx, p, &d_out, &tmp.ptrs));
+    ST_TRY(pool_alloc(ctx, 1, &d_job, &tmp.ptrs));
+    JobDev job = make_grid_job(fw, n, 1, d_out, ctx->tuning, MATH_OVERLAP);
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(d_job, &job, sizeof(job), hipMemcpyHostToDevice, s));
+    ST_TRY(run_jobs(ctx, MATH_OVERLAP, d_job, 1, job.n_items, ctx->d_counter));
+    HIP_TRY(hipMemcpyAsync(host_out, d_out, sizeof(double) * p, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return ST_OK;
 }

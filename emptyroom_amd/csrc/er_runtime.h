@@ -142,6 +142,11 @@ int widom_run(Context *ctx, FrameworkDev *fw, uint64_t n_ins, double temperature
 int count_incut_grid(Context *ctx, FrameworkDev *fw, const int n[3], unsigned long long *out);
 int count_incut_widom(Context *ctx, FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
                       unsigned long long *out);
+// Hard-sphere accessibility grid (src/computegridsheet.h:101-165): 1 where no
+// atom image with offsets in [-1, 1]^3 lies closer than sigma_ij, else 0;
+// grid points f = i / (N - 1), layout k + j*nz + i*ny*nz.
+int accessibility_grid(Context *ctx, const Particle *atoms, int n_atoms, const double t[9],
+                       const int n[3], double *host_out);
 
 // Device buffers and job descriptors of one Widom job.
 struct WidomBuffers {

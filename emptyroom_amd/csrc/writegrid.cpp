@@ -5,8 +5,11 @@
 //
 //   writegrid <structure> <adsorbate>
 //
-// simulation.input may carry the extra key "GridPrecision fp32|fp64"
-// (default fp64, the reference's arithmetic); the reference ignores it.
+// simulation.input may carry the extra keys "GridPrecision fp32|fp64"
+// (default fp64, the reference's arithmetic), "GridOutputFormat bin" (the
+// binary side-format, data/grids/<fw>_<ads>_<ff>.grid) and "AccessibilityGrid 0|1"
+// (the hard-sphere accessibility branch the reference hardcodes off,
+// src/writegrid.cu:57); the reference's scanner ignores both.
 #include <chrono>
 #include <cstdio>
 #include <cstdlib>
@@ -41,6 +44,7 @@ int main(int argc, char *argv[]) {
         printf("%s", msg.c_str());
         exit(EXIT_FAILURE);
     }
+    if (p.accessibility) printf("\n\nPriting accessibility grid based on hard-sphere model...\n\n");
     if (p.verbose) printf("Read simulation.input, .uc file, forcefield and framework\n");
 
     char outputfilename[512];
@@ -57,16 +61,28 @@ int main(int argc, char *argv[]) {
     const int n[3] = {p.N_x, p.N_y, p.N_z};
     char gridfilename[512];
     FILE *gridfile = nullptr;
+    if (p.accessibility && p.gridoutputformat != "cube") {
+        // the reference opens no grid file for this combination
+        // (src/writegrid.cu:144-156) and then writes through it
+        printf("The accessibility grid is written in cube format only\n");
+        exit(EXIT_FAILURE);
+    }
     if (p.gridoutputformat == "txt") {
         snprintf(gridfilename, sizeof(gridfilename), "data/grids/%s_%s_%s.txt",
                  r.fw.name.c_str(), p.adsorbate.c_str(), r.ff.name.c_str());
         gridfile = fopen(gridfilename, "w");
         if (gridfile) write_txt_header(gridfile, n);
     } else if (p.gridoutputformat == "cube") {
-        snprintf(gridfilename, sizeof(gridfilename), "data/grids/%s_%s_%s.cube",
+        snprintf(gridfilename, sizeof(gridfilename),
+                 p.accessibility ? "data/grids/%s_%s_%s_accessibility.cube"
+                                 : "data/grids/%s_%s_%s.cube",
                  r.fw.name.c_str(), p.adsorbate.c_str(), r.ff.name.c_str());
         gridfile = fopen(gridfilename, "w");
         if (gridfile) write_cube_header(gridfile, n, r.fw.t_matrix);
+    } else if (p.gridoutputformat == "bin") {  // extension: binary side-format
+        snprintf(gridfilename, sizeof(gridfilename), "data/grids/%s_%s_%s.grid",
+                 r.fw.name.c_str(), p.adsorbate.c_str(), r.ff.name.c_str());
+        gridfile = fopen(gridfilename, "wb");
     } else {
         printf("Grid output format must be txt or cube\n");
         exit(EXIT_FAILURE);
@@ -89,10 +105,26 @@ int main(int argc, char *argv[]) {
     if (er_context_create(0, 0, &ctx) != ER_OK) return gpu_fail("er_context_create");
     double t[9];
     flat_t(r.fw, t);
+    const size_t npts = (size_t)n[0] * n[1] * n[2];
+    if (p.accessibility) {
+        printf("Starting...\n");
+        std::vector<double> acc(npts);
+        if (er_accessibility_grid(ctx, reinterpret_cast<const er_particle *>(r.atoms.data()),
+                                  r.fw.noatoms, t, n, acc.data()) != ER_OK)
+            return gpu_fail("er_accessibility_grid");
+        if (write_accessibility_values(gridfile, acc.data(), n) != er::ST_OK) {
+            printf("Error writing %s\n", gridfilename);
+            exit(EXIT_FAILURE);
+        }
+        fprintf(outputfile, "    Time to write grid: %f s\n", now_s() - t0);
+        er_context_destroy(ctx);
+        fclose(outputfile);
+        fclose(gridfile);
+        return 0;
+    }
     if (er_framework_upload(ctx, reinterpret_cast<const er_particle *>(r.atoms.data()),
                             r.fw.noatoms, t, p.rep, p.r_cutoff_squared, &fw) != ER_OK)
         return gpu_fail("er_framework_upload");
-    const size_t npts = (size_t)n[0] * n[1] * n[2];
     std::vector<double> e64;
     std::vector<float> e32;
     int st;
@@ -106,7 +138,15 @@ int main(int argc, char *argv[]) {
     if (st != ER_OK) return gpu_fail("er_grid_compute");
     if (p.verbose) printf("Grid computed; writing %zu points\n", npts);
     const int threads = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (write_grid_values(gridfile, p.gridoutputformat, e64.empty() ? nullptr : e64.data(),
+    if (p.gridoutputformat == "bin") {
+        fclose(gridfile);
+        gridfile = nullptr;
+        if (write_grid_bin(gridfilename, n, t, e64.empty() ? nullptr : e64.data(),
+                           e32.empty() ? nullptr : e32.data(), &msg) != er::ST_OK) {
+            printf("%s\n", msg.c_str());
+            exit(EXIT_FAILURE);
+        }
+    } else if (write_grid_values(gridfile, p.gridoutputformat, e64.empty() ? nullptr : e64.data(),
                           e32.empty() ? nullptr : e32.data(), n, threads) != er::ST_OK) {
         printf("Error writing %s\n", gridfilename);
         exit(EXIT_FAILURE);
@@ -117,6 +157,6 @@ int main(int argc, char *argv[]) {
     er_framework_release(fw);
     er_context_destroy(ctx);
     fclose(outputfile);
-    fclose(gridfile);
+    if (gridfile) fclose(gridfile);
     return 0;
 }

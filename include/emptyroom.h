@@ -291,6 +291,31 @@ void er_host_run_release(er_host_run *run);
 int er_write_grid_file(const char *path, const char *format, const double *e64, const float *e32,
                        const int n[3], const double t_matrix[9], int threads);
 
+/* Hard-sphere accessibility grid, ComputeGridSheetAccessibleOrNot
+ * (src/computegridsheet.h:101-165): out[k + j*nz + i*ny*nz] = 1.0 where no
+ * atom image with offsets in [-1, 1]^3 (whatever the replication factors)
+ * lies strictly closer than sigma_ij (atoms[].sig2, mixed with the guest),
+ * else 0.0; points f = i / (N - 1).  The reference sizes this grid as
+ * N_i = ceil(1 / resolution) (src/writegrid.cu:104-108). */
+int er_accessibility_grid(er_context *ctx, const er_particle *atoms, int n_atoms,
+                          const double t_matrix[9], const int n[3], double *out);
+/* ---- grid files ---------------------------------------------------------
+ * Text grids (src/writegrid.cu:144-147,260-269; the format er_write_grid_file
+ * and the drop-in writegrid write) are read as src/gcmc.cc:286-346 imports
+ * them: header "N_x N_y N_z ...", then N_x*N_y lines of N_z values (one
+ * z-pencil per line), layout k + j*nz + i*ny*nz; a short or missing line is
+ * an error.  No reference counterpart: a binary side-format (128-byte header
+ * "ERGRID01", version, dtype, N_x, N_y, N_z, t_matrix; then raw
+ * little-endian FP32 or FP64 values in the same layout) that the readers
+ * recognise by its magic. */
+int er_grid_file_info(const char *path, int n[3]);
+int er_read_grid_file(const char *path, const int n_expected[3], double *out);
+int er_write_grid_binary(const char *path, const double *e64, const float *e32, const int n[3],
+                         const double t_matrix[9]);
+/* A text or binary grid file as a resident FP32 grid (for pocket blocking and
+ * grid-interpolated Widom). */
+int er_grid_load(er_context *ctx, const char *path, er_grid **out);
+
 /* Replication factors (image half-ranges) of julia_codes/framework.jl:162-211
  * for a row-major fractional->Cartesian matrix: per axis, 1 plus the number
  * of cell-vector doublings until the supercell centre is at least `cutoff`

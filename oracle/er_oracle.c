@@ -246,6 +246,36 @@ void orc_lj_points(const double t[9], const double *atoms, int n, const int rep[
     free(img);
 }
 
+/* src/computegridsheet.h:101-165: a point is accessible (1) unless some atom
+   image with offsets in [-1, 1]^3 -- whatever the replication factors -- lies
+   strictly closer than sigma_ij (r^2 < sig2, no clamp); grid points as the
+   energy grid (f = i / (N - 1)). */
+void orc_accessibility_grid(const double t[9], const double *atoms, int n, int nx, int ny,
+                            int nz, double *out, int nthreads) {
+    const int rep1[3] = {1, 1, 1};
+    double *img = build_images(t, atoms, n, rep1);
+    if (!img) return;
+    const int64_t nimg = n_images(rep1);
+    set_threads(nthreads);
+    const int64_t total = (int64_t)nx * ny * nz;
+#pragma omp parallel for schedule(dynamic, 64)
+    for (int64_t p = 0; p < total; p++) {
+        const int i = (int)(p / ((int64_t)ny * nz));
+        const int j = (int)((p / nz) % ny), k = (int)(p % nz);
+        double x, y, z;
+        frac_to_cart(t, 1.0 * i / (nx - 1), 1.0 * j / (ny - 1), 1.0 * k / (nz - 1), &x, &y, &z);
+        int overlap = 0;
+        int64_t q = 0;
+        for (int64_t m = 0; m < nimg && !overlap; m++)
+            for (int a = 0; a < n; a++, q++) {
+                const double dx = x - img[3 * q], dy = y - img[3 * q + 1], dz = z - img[3 * q + 2];
+                if (dx * dx + dy * dy + dz * dz < atoms[5 * a + 4]) overlap = 1;
+            }
+        out[p] = overlap ? 0.0 : 1.0;
+    }
+    free(img);
+}
+
 int64_t orc_count_incut_grid(const double t[9], const double *atoms, int n, const int rep[3],
                              double rc2, int nx, int ny, int nz,
                              int i_begin, int i_end, int i_step, int nthreads) {

@@ -65,6 +65,9 @@ void orc_lj_grid(const double t[9], const double *atoms, int n, const int rep[3]
                  int i_begin, int i_end, int i_step, double *out, int nthreads);
 
 /* Energies at m arbitrary fractional points (m x 3). */
+/* Hard-sphere accessibility grid, src/computegridsheet.h:101-165 (1 / 0). */
+void orc_accessibility_grid(const double t[9], const double *atoms, int n, int nx, int ny,
+                            int nz, double *out, int nthreads);
 void orc_lj_points(const double t[9], const double *atoms, int n, const int rep[3],
                    double rc2, double min_r2, const double *f, int64_t m,
                    double *out, int nthreads);

@@ -44,6 +44,8 @@ def lib():
             [ctypes.c_int] * 6 + [_D, ctypes.c_int]
         L.orc_lj_points.argtypes = [_D, _D, ctypes.c_int, _I, ctypes.c_double, ctypes.c_double,
                                     _D, ctypes.c_int64, _D, ctypes.c_int]
+        L.orc_accessibility_grid.argtypes = [_D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                             ctypes.c_int, _D, ctypes.c_int]
         L.orc_count_incut_grid.argtypes = [_D, _D, ctypes.c_int, _I, ctypes.c_double] + \
             [ctypes.c_int] * 6 + [ctypes.c_int]
         L.orc_count_incut_grid.restype = ctypes.c_int64
@@ -138,6 +140,15 @@ def lj_points(t, atoms, rep, rc2, min_r2, f, nthreads=0):
     lib().orc_lj_points(_d(_c(t)), _d(atoms), len(atoms), _rep(rep), rc2, min_r2, _d(f), len(f),
                         _d(out), nthreads)
     return out
+
+
+def accessibility_grid(t, atoms, n, nthreads=0):
+    """Hard-sphere accessibility (1 accessible / 0 overlap), images [-1, 1]^3."""
+    atoms = _c(atoms).reshape(-1, 5)
+    out = np.empty(int(n[0]) * int(n[1]) * int(n[2]))
+    lib().orc_accessibility_grid(_d(_c(t)), _d(atoms), len(atoms), int(n[0]), int(n[1]),
+                                 int(n[2]), _d(out), nthreads)
+    return out.reshape(int(n[0]), int(n[1]), int(n[2]))
 
 
 def count_incut_grid(t, atoms, rep, rc2, n, sheets=None, nthreads=0):

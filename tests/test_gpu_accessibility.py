@@ -1,0 +1,86 @@
+"""Hard-sphere accessibility grid (src/computegridsheet.h:101-165) on the GPU
+against the oracle: 1/0 per point must agree exactly (the overlap test
+r^2 < sigma^2 is decided in FP64 on both sides; no test point lies within
+rounding of a sphere surface for these seeded inputs)."""
+import os
+import re
+import shutil
+import subprocess
+
+import numpy as np
+import pytest
+
+from emptyroom_amd import api, synth
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+HERE = os.path.dirname(os.path.abspath(__file__))
+
+
+@pytest.mark.parametrize("seed,cell,n_atoms,res", [
+    (1, (10.0, 11.0, 12.0, 85.0, 95.0, 100.0), 40, 0.05),
+    (2, (6.0, 7.5, 5.5, 70.0, 110.0, 65.0), 12, 0.04),
+    (3, (18.0, 19.0, 20.0, 80.0, 100.0, 95.0), 150, 0.025),
+])
+def test_accessibility_matches_oracle(ctx, seed, cell, n_atoms, res):
+    print(f"accessibility seed={seed} cell={cell}", flush=True)
+    f = synth.small_framework(seed, n_atoms=n_atoms, cell=cell)
+    a = f.particles()
+    n = [int(np.ceil(1.0 / res))] * 3  # src/writegrid.cu:104-108
+    ref = oracle.accessibility_grid(f.t, a, n)
+    got = api.accessibility_grid(ctx, a[:, :3], a[:, 3], a[:, 4], f.t, n)
+    assert got.shape == ref.shape
+    assert np.array_equal(got, ref), int(np.sum(got != ref))
+    assert 0.0 < ref.mean() < 1.0
+
+
+def test_accessibility_empty_framework(ctx):
+    t = np.diag([10.0, 10.0, 10.0])
+    got = api.accessibility_grid(ctx, np.zeros((0, 3)), np.zeros(0), np.zeros(0), t, [4, 4, 4])
+    assert np.all(got == 1.0)
+
+
+def test_writegrid_accessibility_cube(tmp_path):
+    """The drop-in's AccessibilityGrid branch: N = ceil(1/res) per axis,
+    <fw>_<ads>_<ff>_accessibility.cube with "%d " values, 6 per line."""
+    print("writegrid accessibility IRMOF-1 Xe", flush=True)
+    root = tmp_path / "run"
+    shutil.copytree(os.path.join(HERE, "golden", "irmof1"), root,
+                    copy_function=shutil.copyfile)
+    for dp, dn, fn in os.walk(root):  # the checkout may be read-only
+        os.chmod(dp, 0o755)
+        for name in fn:
+            os.chmod(os.path.join(dp, name), 0o644)
+    (root / "output_files").mkdir()
+    (root / "data" / "grids").mkdir()
+    sim = (root / "simulation.input").read_text()
+    sim = re.sub(r"^GridOutputFormat .*$", "GridOutputFormat cube", sim, flags=re.M)
+    sim = re.sub(r"^GridResolution .*$", "GridResolution 0.05", sim, flags=re.M)
+    sim += "\nAccessibilityGrid 1\n"
+    (root / "simulation.input").write_text(sim)
+    out = subprocess.run([os.path.join(api.BIN_DIR, "writegrid"), "IRMOF-1_clean", "Xe"],
+                         cwd=root, capture_output=True, text=True, timeout=300)
+    assert out.returncode == 0, out.stdout
+    assert "accessibility grid" in out.stdout
+    lines = (root / "data" / "grids" / "IRMOF-1_clean_Xe_UFF_accessibility.cube").read_text()
+    body = lines.split("\n")[6:]
+    vals = np.array([int(v) for line in body for v in line.split()])
+    n = 20
+    assert vals.size == n ** 3
+    hr = api.HostRun(str(root), "IRMOF-1_clean", "Xe", 0)
+    a = hr.particles()
+    t = np.asarray(hr.info().t_matrix).reshape(3, 3)
+    hr.close()
+    ref = oracle.accessibility_grid(t, a, [n, n, n]).reshape(-1)
+    assert np.array_equal(vals.astype(np.float64), ref)
+
+
+def test_accessibility_images_limited_to_one_cell(ctx):
+    """Sheared cell whose only overlapping image is three cells away: the
+    point stays accessible, as in the reference's [-1, 1]^3 loop."""
+    t = np.column_stack([[6.0, 0.0, 0.0], [25.0, 1.0, 0.0], [0.0, 0.0, 20.0]])
+    atoms = np.array([[0.0, 0.0, 0.0, 50.0, 9.0]])
+    n = [3, 3, 3]
+    got = api.accessibility_grid(ctx, atoms[:, :3], atoms[:, 3], atoms[:, 4], t, n)
+    assert np.array_equal(got, oracle.accessibility_grid(t, atoms, n))
+    assert got[1, 1, 0] == 1.0

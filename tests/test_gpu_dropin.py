@@ -84,3 +84,26 @@ def test_henry_irmof1(tmp_path):
                                               info.density)
     assert abs(kh - kh_ref) / kh_ref < 1e-4
     assert abs(e_k - e_ref) < 1e-3 * max(1.0, abs(e_ref))
+
+
+def test_writegrid_binary_output(tmp_path):
+    """GridOutputFormat bin: the .grid side-format holds exactly the FP64 grid
+    the library computes for the same inputs."""
+    print("writegrid IRMOF-1 Kr bin", flush=True)
+    d = workdir(tmp_path, {"GridOutputFormat": "bin", "GridResolution": "0.5"})
+    p = subprocess.run([os.path.join(api.BIN_DIR, "writegrid"), "IRMOF-1_clean", "Kr"], cwd=d,
+                       capture_output=True, text=True, timeout=300)
+    assert p.returncode == 0, p.stdout
+    got = api.read_grid_file(str(d / "data" / "grids" / "IRMOF-1_clean_Kr_UFF.grid"))
+    hr = api.HostRun(str(d), "IRMOF-1_clean", "Kr", 0)
+    info = hr.info()
+    a = hr.particles()
+    hr.close()
+    n = list(info.grid_n)
+    assert list(got.shape) == n
+    with api.Context(0) as c:
+        fw = api.Framework(c, a[:, :3], a[:, 3], a[:, 4], np.asarray(info.t_matrix).reshape(3, 3),
+                           list(info.rep), info.r_cutoff_squared)
+        ref = api.compute_grid(c, fw, n, api.ER_FP64)
+        fw.close()
+    assert np.array_equal(got, ref)

@@ -110,3 +110,20 @@ def test_batch_matches_oracle_per_grid(ctx):
         assert n == ref_n
         assert np.array_equal(h.download(), ref_blocked.astype(np.float32))
         h.close()
+
+
+def test_pockets_on_loaded_grid_file(ctx, tmp_path):
+    """A text grid file (the reference's txt format) loaded as a resident
+    FP32 grid: pockets and blocked energies as the oracle on the same values."""
+    print("pockets on a loaded txt grid", flush=True)
+    g = random_grid(31, (21, 19, 17), 0.35).astype(np.float64)
+    path = tmp_path / "g.txt"
+    api.write_grid_file(str(path), "txt", g, np.diag([10.0, 9.0, 8.0]), threads=2)
+    values = api.read_grid_file(str(path)).astype(np.float32)
+    grid = api.Grid.load(ctx, str(path))
+    assert np.array_equal(grid.download(), values)
+    n, _ = api.pockets_label(ctx, grid, T, block=True)
+    ref_n, ref_blocked, _ = oracle.find_and_block_pockets(values.astype(np.float64), T)
+    assert n == ref_n
+    assert np.array_equal(grid.download(), ref_blocked.astype(np.float32))
+    grid.close()
