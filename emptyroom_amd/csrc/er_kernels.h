@@ -73,6 +73,8 @@ int tiles_occupancy(int math, int ppl, int cap);
 constexpr int kReduceBlocks = 240;
 hipError_t launch_widom_partials(const double *e, long long n, double temperature,
                                  double2 *partials, hipStream_t s);
+// Fixed-order sum of nblk partials (out[0] = sum_b, out[1] = sum_Eb).
+hipError_t launch_widom_final_n(const double2 *partials, int nblk, double *out, hipStream_t s);
 // out[0] = sum_b, out[1] = sum_Eb: fixed-order sum of the kReduceBlocks partials.
 hipError_t launch_widom_final(const double2 *partials, double *out, hipStream_t s);
 // Sum of n uint32 counts into *out (uint64, must be zero).

@@ -550,6 +550,36 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                     acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
             }
         }
+        if (MATH == MATH_MIXED) {
+            // Near pairs of an FP32 grid or Widom run: the expanded FP64 r^2 of
+            // the far pairs (|a| <= r_FP64 + R keeps it exact to ~1e-14 A^2),
+            // the reference's clamp and strict cutoff, u = 1/r^2 from the FP32
+            // reciprocal and one Newton step (~1e-14 relative), and FP64 sums
+            // of u^3 and u^6 scaled by 4 eps sigma^6 / sigma^12 per segment.
+            const double sig6 = P.sig2 * P.sig2 * P.sig2;
+            const double c6 = P.eps4 * sig6 * sig6, c3 = -P.eps4 * sig6;
+            double b3[PPL], b6[PPL];
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) b3[q] = b6[q] = 0.0;
+            for (int k = nb; k < ne; ++k) {
+                const double4 A = Ln[k];
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) {
+                    double r2 = fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
+                    r2 = r2 > min_r2 ? r2 : min_r2;  // src/computegridsheet.h:77
+                    double y = (double)__builtin_amdgcn_rcpf((float)r2);
+                    y = fma(fma(-r2, y, 1.0), y, y);
+                    double y3 = y * y * y;
+                    y3 = r2 < rc2 ? y3 : 0.0;  // src/computegridsheet.h:79
+                    b3[q] += y3;
+                    b6[q] = fma(y3, y3, b6[q]);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) acc[q] = fma(c6, b6[q], fma(c3, b3[q], acc[q]));
+            continue;
+        }
+        // FP64 grids: the reference's arithmetic on displacements
         const double sig2 = P.sig2, eps4 = P.eps4;
         for (int k = nb; k < ne; ++k) {
             const double4 A = Ln[k];
@@ -561,7 +591,7 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                 const bool in = r2 < rc2;        // src/computegridsheet.h:79
                 double y = (double)__builtin_amdgcn_rcpf((float)r2);
                 y = fma(fma(-r2, y, 1.0), y, y);
-                if (MATH == MATH_FP64) y = fma(fma(-r2, y, 1.0), y, y);
+                y = fma(fma(-r2, y, 1.0), y, y);
                 const double s2 = sig2 * y;
                 const double s6 = s2 * s2 * s2;
                 const double e = eps4 * fma(s6, s6, -s6);
@@ -875,15 +905,28 @@ __global__ __launch_bounds__(kWG) void k_widom_partials(const double *e, long lo
                                    (s_eb[0] + s_eb[1]) + (s_eb[2] + s_eb[3]));
 }
 
-__global__ void k_widom_final(const double2 *partials, int nblk, double *out) {
-    if (threadIdx.x != 0 || blockIdx.x != 0) return;
+// One block of kWG threads: thread t sums partials t, t + kWG, ... in order,
+// then a fixed shuffle tree per wave and the wave sums in order -- the same
+// summation order on every run.
+__global__ __launch_bounds__(kWG) void k_widom_final(const double2 *partials, int nblk,
+                                                     double *out) {
+    __shared__ double s_b[kWG / 64], s_eb[kWG / 64];
     double sb = 0.0, seb = 0.0;
-    for (int g = 0; g < nblk; ++g) {
+    for (int g = threadIdx.x; g < nblk; g += kWG) {
         sb += partials[g].x;
         seb += partials[g].y;
     }
-    out[0] = sb;
-    out[1] = seb;
+    sb = wave_sum(sb);
+    seb = wave_sum(seb);
+    if ((threadIdx.x & 63) == 0) {
+        s_b[threadIdx.x >> 6] = sb;
+        s_eb[threadIdx.x >> 6] = seb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        out[0] = (s_b[0] + s_b[1]) + (s_b[2] + s_b[3]);
+        out[1] = (s_eb[0] + s_eb[1]) + (s_eb[2] + s_eb[3]);
+    }
 }
 
 __global__ __launch_bounds__(kWG) void k_sum_u32(const uint32_t *v, long long n,
@@ -1016,7 +1059,12 @@ hipError_t launch_widom_partials(const double *e, long long n, double temperatur
 }
 
 hipError_t launch_widom_final(const double2 *partials, double *out, hipStream_t s) {
-    hipLaunchKernelGGL(k_widom_final, dim3(1), dim3(64), 0, s, partials, kReduceBlocks, out);
+    hipLaunchKernelGGL(k_widom_final, dim3(1), dim3(kWG), 0, s, partials, kReduceBlocks, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_widom_final_n(const double2 *partials, int nblk, double *out, hipStream_t s) {
+    hipLaunchKernelGGL(k_widom_final, dim3(1), dim3(kWG), 0, s, partials, nblk, out);
     return hipGetLastError();
 }
 

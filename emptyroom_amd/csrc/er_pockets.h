@@ -45,7 +45,11 @@ hipError_t launch_cc_segments(const CCGrid *grids, int ng, long long total_block
                               hipStream_t s);
 // Overwrite the voxels of pocket segments with 1e13 (grids with pocket != null).
 hipError_t launch_cc_block(const CCGrid *grids, int ng, long long total_blocks, hipStream_t s);
+// Grid-interpolated Widom with fused Boltzmann sums: partials (kTriBlocks
+// entries) receive fixed-order per-block (sum b, sum E b); e_out nullable.
+constexpr int kTriBlocks = 4096;
 hipError_t launch_widom_grid(const float *g, const int n[3], uint64_t seed, uint32_t stream,
-                             uint64_t nins, double *e_out, hipStream_t s);
+                             uint64_t nins, double temperature, double *e_out, double2 *partials,
+                             hipStream_t s);
 
 }  // namespace er

@@ -470,20 +470,43 @@ __device__ __forceinline__ double trilinear(const float *__restrict__ g, int nx,
     return c0 * (1 - zd) + c1 * zd;
 }
 
-__global__ void k_widom_grid(const float *__restrict__ g, int nx, int ny, int nz, uint64_t seed,
-                             uint32_t stream, uint64_t n, double *__restrict__ e_out) {
-    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < n;
-         i += (uint64_t)gridDim.x * blockDim.x) {
+// Grid-interpolated insertions with the Boltzmann sums fused in: block b of
+// kTriBlocks takes the contiguous insertion range [b n / B, (b+1) n / B),
+// its threads stride through it, and the block writes one (sum b, sum E b)
+// partial in a fixed order, so the result does not depend on scheduling.
+// e_out (nullable) receives the energy of every insertion.
+__global__ __launch_bounds__(kWG) void k_widom_grid(const float *__restrict__ g, int nx, int ny,
+                                                    int nz, uint64_t seed, uint32_t stream,
+                                                    uint64_t n, double temperature,
+                                                    double *__restrict__ e_out,
+                                                    double2 *__restrict__ partials) {
+    __shared__ double s_b[kWG / 64], s_eb[kWG / 64];
+    const uint64_t B = gridDim.x, b = blockIdx.x;
+    const uint64_t lo = b * n / B, hi = (b + 1) * n / B;
+    double sb = 0.0, seb = 0.0;
+    for (uint64_t i = lo + threadIdx.x; i < hi; i += kWG) {
         double f[3];
         widom_point(seed, stream, i, f);
-        e_out[i] = trilinear(g, nx, ny, nz, f[0] - floor(f[0]), f[1] - floor(f[1]),
-                             f[2] - floor(f[2]));
+        const double E = trilinear(g, nx, ny, nz, f[0] - floor(f[0]), f[1] - floor(f[1]),
+                                   f[2] - floor(f[2]));
+        if (e_out) e_out[i] = E;
+        const double bz = exp(-E / temperature);
+        sb += bz;
+        seb += (bz == 0.0) ? 0.0 : E * bz;
     }
-}
-
-int blocks_for(long long n) {
-    long long b = (n + 255) / 256;
-    return (int)(b < 1 ? 1 : (b > 8192 ? 8192 : b));
+#pragma unroll
+    for (int o = 32; o >= 1; o >>= 1) {
+        sb += __shfl_xor(sb, o, 64);
+        seb += __shfl_xor(seb, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        s_b[threadIdx.x >> 6] = sb;
+        s_eb[threadIdx.x >> 6] = seb;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partials[b] = make_double2((s_b[0] + s_b[1]) + (s_b[2] + s_b[3]),
+                                   (s_eb[0] + s_eb[1]) + (s_eb[2] + s_eb[3]));
 }
 
 }  // namespace
@@ -534,10 +557,11 @@ hipError_t launch_cc_block(const CCGrid *grids, int ng, long long total_blocks, 
 }
 
 hipError_t launch_widom_grid(const float *g, const int n[3], uint64_t seed, uint32_t stream,
-                             uint64_t nins, double *e_out, hipStream_t s) {
+                             uint64_t nins, double temperature, double *e_out, double2 *partials,
+                             hipStream_t s) {
     if (nins == 0) return hipSuccess;
-    hipLaunchKernelGGL(k_widom_grid, dim3(blocks_for((long long)nins)), dim3(256), 0, s, g, n[0],
-                       n[1], n[2], seed, stream, nins, e_out);
+    hipLaunchKernelGGL(k_widom_grid, dim3(kTriBlocks), dim3(kWG), 0, s, g, n[0], n[1], n[2], seed,
+                       stream, nins, temperature, e_out, partials);
     return hipGetLastError();
 }
 

@@ -355,12 +355,13 @@ int widom_grid_run(Context *ctx, GridDev *g, uint64_t n_ins, double temperature,
     }
     HIP_TRY(hipSetDevice(ctx->device));
     Scratch sc{ctx, {}};
-    double *e = sc.get<double>(n_ins);
-    if (!e) return ST_NOMEM;
+    double *e = energies_host ? sc.get<double>(n_ins) : nullptr;
+    double2 *partials = sc.get<double2>(kTriBlocks);
+    if ((energies_host && !e) || !partials) return ST_NOMEM;
     hipStream_t s = ctx->stream;
-    HIP_TRY(launch_widom_grid((const float *)g->d_data, g->n, seed, 1u, n_ins, e, s));
-    HIP_TRY(launch_widom_partials(e, (long long)n_ins, temperature, ctx->d_partials, s));
-    HIP_TRY(launch_widom_final(ctx->d_partials, ctx->d_sums, s));
+    HIP_TRY(launch_widom_grid((const float *)g->d_data, g->n, seed, 1u, n_ins, temperature, e,
+                              partials, s));
+    HIP_TRY(launch_widom_final_n(partials, kTriBlocks, ctx->d_sums, s));
     double sums[2];
     HIP_TRY(hipMemcpyAsync(sums, ctx->d_sums, sizeof(sums), hipMemcpyDeviceToHost, s));
     if (energies_host)

@@ -395,13 +395,38 @@ void choose_grid_tile(const FrameworkDev *fw, const int n[3], const Tuning &tu, 
     tile[2] = tu.tile[2];
 }
 
+// Bins of near-cubic shape (per perpendicular width) whose mean occupancy is
+// at most widom_per_bin: the tile kernel takes a bin's insertions 256 at a
+// time, so a mean just below a multiple of 256 keeps the last chunk full
+// while bins above the mean stay rare.  Refines the rounded counts: adds bins
+// along the axis with the widest bins until the mean fits, then removes any
+// bin whose removal keeps it fitting.
 void choose_widom_bins(const FrameworkDev *fw, uint64_t n_ins, const Tuning &tu, int nb[3]) {
     double w[3];
     for (int d = 0; d < 3; d++) w[d] = perp_width(fw->inv_t, d);
-    double target = (double)n_ins / std::max(tu.widom_per_bin, 1);
+    const double per = (double)std::max(tu.widom_per_bin, 1);
+    double target = (double)n_ins / per;
     target = std::min(std::max(target, 1.0), (double)(1 << 21));
     const double k = std::cbrt(target / (w[0] * w[1] * w[2]));
     for (int d = 0; d < 3; d++) nb[d] = std::max(1, std::min(1 << 10, (int)std::lround(w[d] * k)));
+    auto mean = [&]() { return (double)n_ins / ((double)nb[0] * nb[1] * nb[2]); };
+    while (mean() > per) {
+        int d = -1;
+        for (int e = 0; e < 3; e++)
+            if (nb[e] < (1 << 10) && (d < 0 || w[e] / nb[e] > w[d] / nb[d])) d = e;
+        if (d < 0) break;
+        nb[d]++;
+    }
+    for (bool changed = true; changed;) {
+        changed = false;
+        int d = -1;  // narrowest bins first
+        for (int e = 0; e < 3; e++)
+            if (nb[e] > 1 && (d < 0 || w[e] / nb[e] < w[d] / nb[d])) d = e;
+        if (d < 0) break;
+        nb[d]--;
+        if (mean() <= per) changed = true;
+        else nb[d]++;
+    }
 }
 
 // Tiles per region per axis: about region_len Angstrom along each lattice

@@ -46,7 +46,7 @@ struct Tuning {
     int ppl = 1;                 // points per lane (1 or 2)
     int cap = 1024;              // LDS list capacity (records of 32 B)
     int tile[3] = {8, 8, 4};     // grid tile in points, product = 256 * ppl
-    int widom_per_bin = 512;     // target insertions per Widom bin
+    int widom_per_bin = 480;     // largest mean insertions per Widom bin
     double s2_switch = 0.444;    // (sigma/r)^2 above which a pair takes the FP64 path (r < 1.5 sigma)
     double region_len = 4.0;     // edge of a region of grid tiles in Angstrom (0: no regions)
 };

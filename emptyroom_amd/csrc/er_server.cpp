@@ -214,6 +214,18 @@ void Server::finish(Req *r) {
     done_cv_.notify_all();
 }
 
+void Server::fail(Req *r) {
+    delete r->fw;
+    r->fw = nullptr;
+    delete r->grid;
+    r->grid = nullptr;
+    for (void *p : r->owned) ctx_->pool.free(p);
+    r->owned.clear();
+    r->info.state = RQ_FAILED;
+    r->info.t_done = now_s();
+    done_cv_.notify_all();
+}
+
 void Server::run() {
     (void)hipSetDevice(ctx_->device);
     while (!stop_) {
@@ -238,15 +250,27 @@ void Server::run() {
                     double *sums = nullptr;
                     part = (double2 *)ctx_->pool.alloc(sizeof(double2) * kReduceBlocks);
                     sums = (double *)ctx_->pool.alloc(2 * sizeof(double));
+                    if (part) r->owned.push_back(part);
+                    if (sums) r->owned.push_back(sums);
+                    hipError_t e = hipErrorOutOfMemory;
                     if (part && sums) {
-                        r->owned.push_back(part);
-                        r->owned.push_back(sums);
-                        (void)launch_widom_partials(r->wb.e, (long long)r->spec.n_ins,
-                                                    r->spec.temperature, part, s);
-                        (void)launch_widom_final(part, sums, s);
-                        (void)hipMemcpyAsync(r->h_sums, sums, 2 * sizeof(double),
-                                             hipMemcpyDeviceToHost, s);
-                        (void)hipEventRecord(r->reduced, s);
+                        e = launch_widom_partials(r->wb.e, (long long)r->spec.n_ins,
+                                                  r->spec.temperature, part, s);
+                        if (e == hipSuccess) e = launch_widom_final(part, sums, s);
+                        if (e == hipSuccess)
+                            e = hipMemcpyAsync(r->h_sums, sums, 2 * sizeof(double),
+                                               hipMemcpyDeviceToHost, s);
+                        if (e == hipSuccess) e = hipEventRecord(r->reduced, s);
+                    }
+                    if (e != hipSuccess) {
+                        // no complete reduction in flight: the request fails
+                        // (an event that was never recorded would read as
+                        // complete); wait for any part already enqueued
+                        (void)hipStreamSynchronize(s);
+                        r->info.status = e == hipErrorOutOfMemory ? ST_NOMEM : ST_HIP;
+                        r->info.error = "Widom reduction could not be enqueued";
+                        fail(r);
+                        drop = true;
                     }
                     r->reducing = true;
                     progress = true;

@@ -81,6 +81,7 @@ class Server {
     void run();
     bool admit_one(Req *r);
     void finish(Req *r);
+    void fail(Req *r);  // state RQ_FAILED (status / error already set); frees its buffers
     size_t estimate_bytes(const RequestSpec &s) const;
 
     Context *ctx_;

@@ -77,7 +77,7 @@ typedef struct er_tuning {
     int points_per_lane;   /* 1 or 2 */
     int lds_records;       /* LDS candidate-list capacity, 512..4800 records of 32 B */
     int tile[3];           /* grid tile in points, product = 256 * points_per_lane */
-    int widom_per_bin;     /* target insertions per Widom bin */
+    int widom_per_bin;     /* largest mean insertions per Widom bin */
     double s2_switch;      /* (sigma/r)^2 above which an FP32-grid pair uses FP64 */
     double region_len;     /* edge in Angstrom of the tile regions that share one
                               candidate sub-list (0 = every tile scans the whole

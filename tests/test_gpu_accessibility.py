@@ -18,8 +18,8 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 
 
 @pytest.mark.parametrize("seed,cell,n_atoms,res", [
-    (1, (10.0, 11.0, 12.0, 85.0, 95.0, 100.0), 40, 0.05),
-    (2, (6.0, 7.5, 5.5, 70.0, 110.0, 65.0), 12, 0.04),
+    (1, (10.0, 11.0, 12.0, 85.0, 95.0, 100.0), 8, 0.05),
+    (2, (8.0, 9.5, 7.5, 70.0, 110.0, 65.0), 3, 0.04),
     (3, (18.0, 19.0, 20.0, 80.0, 100.0, 95.0), 150, 0.025),
 ])
 def test_accessibility_matches_oracle(ctx, seed, cell, n_atoms, res):
