@@ -527,22 +527,30 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                 for (int q = 0; q < PPL; ++q)
                     acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
             }
+            auto shell_pair = [&](const double4 &A, int q, float &s3, float &s6) {
+                const double r2 = fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
+                const float w = __builtin_amdgcn_rcpf((float)r2);
+                float w3 = w * w * w;
+                w3 = r2 < rc2 ? w3 : 0.0f;  // src/computegridsheet.h:79
+                s3 += w3;
+                s6 = __builtin_fmaf(w3, w3, s6);
+            };
             for (int k0 = sb; k0 < se; k0 += 8) {
                 float a3[PPL], a6[PPL];
 #pragma unroll
                 for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
-                const int k1 = k0 + 8 < se ? k0 + 8 : se;
-                for (int k = k0; k < k1; ++k) {
-                    const double4 A = Ls[-k];
+                if (k0 + 8 <= se) {
 #pragma unroll
-                    for (int q = 0; q < PPL; ++q) {
-                        const double r2 =
-                            fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
-                        const float w = __builtin_amdgcn_rcpf((float)r2);
-                        float w3 = w * w * w;
-                        w3 = r2 < rc2 ? w3 : 0.0f;  // src/computegridsheet.h:79
-                        a3[q] += w3;
-                        a6[q] = __builtin_fmaf(w3, w3, a6[q]);
+                    for (int u = 0; u < 8; ++u) {
+                        const double4 A = Ls[-(k0 + u)];
+#pragma unroll
+                        for (int q = 0; q < PPL; ++q) shell_pair(A, q, a3[q], a6[q]);
+                    }
+                } else {
+                    for (int k = k0; k < se; ++k) {
+                        const double4 A = Ls[-k];
+#pragma unroll
+                        for (int q = 0; q < PPL; ++q) shell_pair(A, q, a3[q], a6[q]);
                     }
                 }
 #pragma unroll
@@ -565,8 +573,9 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                 const double4 A = Ln[k];
 #pragma unroll
                 for (int q = 0; q < PPL; ++q) {
-                    double r2 = fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
-                    r2 = r2 > min_r2 ? r2 : min_r2;  // src/computegridsheet.h:77
+                    // src/computegridsheet.h:77 (r^2 is never NaN, so fmax is the clamp)
+                    const double r2 =
+                        fmax(fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q], min_r2);
                     double y = (double)__builtin_amdgcn_rcpf((float)r2);
                     y = fma(fma(-r2, y, 1.0), y, y);
                     double y3 = y * y * y;
@@ -586,9 +595,8 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
 #pragma unroll
             for (int q = 0; q < PPL; ++q) {
                 const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
-                double r2 = dx * dx + dy * dy + dz * dz;
-                r2 = r2 > min_r2 ? r2 : min_r2;  // src/computegridsheet.h:77
-                const bool in = r2 < rc2;        // src/computegridsheet.h:79
+                const double r2 = fmax(dx * dx + dy * dy + dz * dz, min_r2);  // :77
+                const bool in = r2 < rc2;  // src/computegridsheet.h:79
                 double y = (double)__builtin_amdgcn_rcpf((float)r2);
                 y = fma(fma(-r2, y, 1.0), y, y);
                 y = fma(fma(-r2, y, 1.0), y, y);
