@@ -29,6 +29,7 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector/matrix peak (packed FP32)
+DEFAULT_TUNING_FILE = os.path.join(ROOT, "emptyroom_amd", "tuning", "mi355x_config3.json")
 FLOP_PER_PAIR = 16
 
 
@@ -276,7 +277,9 @@ def measured_traffic(tuning, n_frameworks):
                 if ":" in line and not line.startswith("#"):
                     k, v = line.split(":", 1)
                     kv[k.strip()] = v.strip()
-        if int(kv["frameworks"]) != n_frameworks or json.loads(kv["tuning"]) != tuning:
+        knobs = {k: v for k, v in tuning.items() if k != "source"}
+        filed = {k: v for k, v in json.loads(kv["tuning"]).items() if k != "source"}
+        if int(kv["frameworks"]) != n_frameworks or filed != knobs:
             return None
         return (float(kv["fetch_kib"]) * float(kv["fetch_scale"]) +
                 float(kv["write_kib"]) * float(kv["write_scale"])) * 1024.0
@@ -305,6 +308,11 @@ def main():
     ap.add_argument("--serve-seconds", type=float, default=4.0)
     ap.add_argument("--streams", type=int, default=3)
     ap.add_argument("--items-per-launch", type=int, default=32768)
+    ap.add_argument("--tuning-file", default=DEFAULT_TUNING_FILE,
+                    help="knobs from an autotuner file ('' = library defaults)")
+    ap.add_argument("--autotune", type=int, default=0,
+                    help="autotune on every k-th framework of the workload first (0: off)")
+    ap.add_argument("--save-tuning", default="", help="write the autotuner result here")
     args = ap.parse_args()
     if args.gpus != 1:
         print(json.dumps({"error": "this build runs on one GPU only; --gpus must be 1"}))
@@ -327,6 +335,11 @@ def main():
     fws = make_workload(args.workload, args.frameworks)
     log(f"workload {args.workload}: {len(fws)} frameworks")
     ctx = api.Context(0)
+    tuning_source = "library defaults"
+    if args.tuning_file and os.path.exists(args.tuning_file):
+        from emptyroom_amd import autotune
+        ctx.tuning = autotune.load(args.tuning_file)
+        tuning_source = os.path.relpath(args.tuning_file, ROOT)
     tu = ctx.tuning
     if args.ppl:
         tu.points_per_lane = args.ppl
@@ -342,16 +355,32 @@ def main():
     if args.widom_per_bin:
         tu.widom_per_bin = args.widom_per_bin
     ctx.tuning = tu
-    tuning = dict(ppl=tu.points_per_lane, tile=list(tu.tile), lds_records=tu.lds_records,
-                  region_len=tu.region_len, s2_switch=tu.s2_switch,
-                  widom_per_bin=tu.widom_per_bin)
-    log(f"tuning {tuning}")
     uploaded = []
     for f in fws:
         a = f.particles()
         uploaded.append(api.Framework(ctx, a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2))
     tasks = [dict(fw=u, grid_n=f.grid_n(), precision=api.ER_FP32, n_insertions=f.n_insertions,
                   temperature=f.temperature, seed=f.seed) for f, u in zip(fws, uploaded)]
+    tuned = None
+    if args.autotune:
+        from emptyroom_amd import autotune
+        k = args.autotune
+        log(f"autotuning on every {k}-th framework ({len(fws[::k])} of {len(fws)})")
+        tuned = autotune.autotune(ctx, tasks[::k], [f.t for f in fws[::k]], log=log)
+        log(f"autotuned: step {tuned['start_step_seconds'] * 1e3:.1f} -> "
+            f"{tuned['step_seconds'] * 1e3:.1f} ms on the sample, "
+            f"{tuned['measurements']} configurations in {tuned['search_seconds']:.0f} s")
+        if args.save_tuning:
+            autotune.save(args.save_tuning, tuned,
+                          meta={"workload": args.workload, "sample_every": k,
+                                "frameworks": len(fws), "device": "MI355X (gfx950)"})
+        tuning_source = f"autotuned in this run (every {k}-th framework)"
+    tu = ctx.tuning
+    tuning = dict(ppl=tu.points_per_lane, tile=list(tu.tile), lds_records=tu.lds_records,
+                  region_len=tu.region_len, s2_switch=tu.s2_switch,
+                  widom_per_bin=tu.widom_per_bin,
+                  class_tile=[list(r) for r in tu.class_tile], source=tuning_source)
+    log(f"tuning {tuning}")
     grid_pe = sum(pair_evals(f)[0] for f in fws)
     widom_pe = sum(pair_evals(f)[1] for f in fws)
     n_ins = sum(f.n_insertions for f in fws)

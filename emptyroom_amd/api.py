@@ -49,7 +49,34 @@ class WidomResult(ctypes.Structure):
 class Tuning(ctypes.Structure):
     _fields_ = [("points_per_lane", ctypes.c_int), ("lds_records", ctypes.c_int),
                 ("tile", ctypes.c_int * 3), ("widom_per_bin", ctypes.c_int),
-                ("s2_switch", ctypes.c_double), ("region_len", ctypes.c_double)]
+                ("s2_switch", ctypes.c_double), ("region_len", ctypes.c_double),
+                ("class_spacing", ctypes.c_double * 2), ("class_tile", (ctypes.c_int * 3) * 3)]
+
+    def to_dict(self) -> dict:
+        return dict(points_per_lane=self.points_per_lane, lds_records=self.lds_records,
+                    tile=list(self.tile), widom_per_bin=self.widom_per_bin,
+                    s2_switch=self.s2_switch, region_len=self.region_len,
+                    class_spacing=list(self.class_spacing),
+                    class_tile=[list(r) for r in self.class_tile])
+
+    @classmethod
+    def from_dict(cls, d: dict, base: "Tuning" = None) -> "Tuning":
+        t = cls()
+        if base is not None:
+            ctypes.memmove(ctypes.byref(t), ctypes.byref(base), ctypes.sizeof(cls))
+        for k, v in d.items():
+            if k in ("tile", "class_spacing"):
+                for i, x in enumerate(v):
+                    getattr(t, k)[i] = x
+            elif k == "class_tile":
+                for c, row in enumerate(v):
+                    for i, x in enumerate(row):
+                        t.class_tile[c][i] = x
+            elif k in dict(cls._fields_):
+                setattr(t, k, v)
+            else:
+                raise KeyError(f"unknown tuning field {k}")
+        return t
 
 
 class Task(ctypes.Structure):
@@ -111,6 +138,7 @@ EXPORTS = [
     "er_host_run_write_settings", "er_host_run_release", "er_write_grid_file",
     "er_replication_factors", "er_write_uc_file", "er_accessibility_grid",
     "er_grid_file_info", "er_read_grid_file", "er_write_grid_binary", "er_grid_load",
+    "er_tuning_defaults", "er_grid_tile_for",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -189,6 +217,10 @@ _SIGS = {
     "er_grid_load": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(_P)]),
     "er_accessibility_grid": (ctypes.c_int, [_P, _P, ctypes.c_int, _P,
                                              ctypes.POINTER(ctypes.c_int), _P]),
+    "er_tuning_defaults": (ctypes.c_int, [ctypes.POINTER(Tuning)]),
+    "er_grid_tile_for": (ctypes.c_int, [ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_double),
+                                        ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
+                                        ctypes.POINTER(ctypes.c_int)]),
     "er_replication_factors": (ctypes.c_int, [_P, ctypes.c_double, ctypes.c_int,
                                               ctypes.POINTER(ctypes.c_int)]),
     "er_write_uc_file": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_double,
@@ -269,6 +301,23 @@ class Context:
         a, b = ctypes.c_size_t(), ctypes.c_size_t()
         _check(lib().er_context_memory(self.h, ctypes.byref(a), ctypes.byref(b)))
         return a.value, b.value
+
+
+def default_tuning() -> Tuning:
+    """The library's default kernel knobs (host only, no device needed)."""
+    t = Tuning()
+    _check(lib().er_tuning_defaults(ctypes.byref(t)))
+    return t
+
+
+def grid_tile_for(tuning: Tuning, t_matrix, n: Sequence[int]):
+    """(tile, size class) that `tuning` gives a grid of n points on t_matrix."""
+    tm = np.ascontiguousarray(np.asarray(t_matrix, dtype=np.float64).reshape(9))
+    nn = (ctypes.c_int * 3)(*[int(x) for x in n])
+    tile = (ctypes.c_int * 3)()
+    cls = ctypes.c_int()
+    _check(lib().er_grid_tile_for(ctypes.byref(tuning), _dptr(tm), nn, tile, ctypes.byref(cls)))
+    return tuple(tile), cls.value
 
 
 class Framework:

@@ -72,22 +72,65 @@ int er_context_destroy(er_context *ctx) {
     return ST_OK;
 }
 
-int er_context_get_tuning(const er_context *ctx, er_tuning *out) {
-    if (!ctx || !out) return arg_error("er_context_get_tuning: null argument");
-    const er::Tuning &t = C(ctx)->tuning;
+namespace {
+void tuning_to(const er::Tuning &t, er_tuning *out) {
     out->points_per_lane = t.ppl;
     out->lds_records = t.cap;
     for (int d = 0; d < 3; d++) out->tile[d] = t.tile[d];
     out->widom_per_bin = t.widom_per_bin;
     out->s2_switch = t.s2_switch;
     out->region_len = t.region_len;
+    for (int c = 0; c < 2; c++) out->class_spacing[c] = t.class_spacing[c];
+    for (int c = 0; c < 3; c++)
+        for (int d = 0; d < 3; d++) out->class_tile[c][d] = t.class_tile[c][d];
+}
+}  // namespace
+
+int er_context_get_tuning(const er_context *ctx, er_tuning *out) {
+    if (!ctx || !out) return arg_error("er_context_get_tuning: null argument");
+    tuning_to(C(ctx)->tuning, out);
+    return ST_OK;
+}
+
+int er_tuning_defaults(er_tuning *out) {
+    if (!out) return arg_error("er_tuning_defaults: null argument");
+    tuning_to(er::Tuning(), out);
+    return ST_OK;
+}
+
+namespace {
+er::Tuning tuning_from(const er_tuning *in) {
+    er::Tuning t;
+    t.ppl = in->points_per_lane;
+    t.cap = in->lds_records;
+    for (int d = 0; d < 3; d++) t.tile[d] = in->tile[d];
+    t.widom_per_bin = in->widom_per_bin;
+    t.s2_switch = in->s2_switch;
+    t.region_len = in->region_len;
+    for (int c = 0; c < 2; c++) t.class_spacing[c] = in->class_spacing[c];
+    for (int c = 0; c < 3; c++)
+        for (int d = 0; d < 3; d++) t.class_tile[c][d] = in->class_tile[c][d];
+    return t;
+}
+}  // namespace
+
+int er_grid_tile_for(const er_tuning *tuning, const double t_matrix[9], const int n[3],
+                     int tile_out[3], int *size_class) {
+    if (!tuning || !t_matrix || !n || !tile_out)
+        return arg_error("er_grid_tile_for: null argument");
+    const er::Tuning t = tuning_from(tuning);
+    if (er::check_tuning(t) != ST_OK) return ST_ARG;
+    for (int d = 0; d < 3; d++)
+        if (n[d] < 2) return arg_error("er_grid_tile_for: grid dimensions must be >= 2");
+    er::grid_tile_shape(t_matrix, n, t, tile_out);
+    if (size_class) *size_class = er::grid_size_class(t_matrix, n, t);
     return ST_OK;
 }
 
 int er_context_set_tuning(er_context *ctx, const er_tuning *in) {
     if (!ctx || !in) return arg_error("er_context_set_tuning: null argument");
-    if (in->points_per_lane != 1 && in->points_per_lane != 2)
-        return arg_error("er_context_set_tuning: points_per_lane must be 1 or 2");
+    if (in->points_per_lane != 1 && in->points_per_lane != 2 && in->points_per_lane != 4)
+        return arg_error("er_context_set_tuning: points_per_lane must be 1, 2 or 4");
     if (in->tile[0] < 1 || in->tile[1] < 1 || in->tile[2] < 1 ||
         in->tile[0] * in->tile[1] * in->tile[2] != 256 * in->points_per_lane)
         return arg_error("er_context_set_tuning: tile must multiply to 256 * points_per_lane");
@@ -98,14 +141,10 @@ int er_context_set_tuning(er_context *ctx, const er_tuning *in) {
         return arg_error("er_context_set_tuning: s2_switch must be in (0, 1]");
     if (!(in->region_len >= 0.0) || !(in->region_len <= 1000.0))
         return arg_error("er_context_set_tuning: region_len must be in [0, 1000] Angstrom");
+    const er::Tuning nt = tuning_from(in);
+    if (er::check_tuning(nt) != ST_OK) return ST_ARG;
     er::Context *c = C(ctx);
-    er::Tuning &t = c->tuning;
-    t.ppl = in->points_per_lane;
-    t.cap = in->lds_records;
-    for (int d = 0; d < 3; d++) t.tile[d] = in->tile[d];
-    t.widom_per_bin = in->widom_per_bin;
-    t.s2_switch = in->s2_switch;
-    t.region_len = in->region_len;
+    c->tuning = nt;
     // a tuning with higher occupancy needs more region scratch slices
     return guarded([&]() -> int {
         return er::scratch_ensure(c, c->stream, &c->d_scratch, &c->scratch_wgs);

@@ -18,6 +18,14 @@
 
 #include <float.h>
 
+// No implicit multiply-add contraction in this file: every FMA below is
+// written out, so the arithmetic of a pair (and so every energy) is fixed by
+// the source and does not depend on how the compiler schedules one template
+// instance against another (1, 2 or 4 points per lane give bit-identical
+// energies for the same list).  Coordinates (frac_to_cart) then round like
+// the oracle's and the reference's host code: separate products and sums.
+#pragma clang fp contract(off)
+
 namespace er {
 
 namespace {
@@ -424,6 +432,27 @@ __device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict
     return nseg;
 }
 
+// r^2 from this lane's points to record A (both relative to the tile centre):
+// the expanded form |a|^2 + |p|^2 - 2 p.a with the z, y, x terms in that
+// order.  With XL the lane's points share y and z (grid points that differ
+// only in i, on a cell whose first vector lies along x), so the z and y terms
+// are formed once per record; the result is bit-identical to the per-point
+// form, since the shared terms are the same operations on the same values.
+template <int PPL, bool XL>
+__device__ __forceinline__ void pair_r2(const double4 &A, const double (&mx)[PPL],
+                                        const double (&my)[PPL], const double (&mz)[PPL],
+                                        const double (&p2)[PPL], double (&r2)[PPL]) {
+    if (XL) {
+        const double yz = fma(my[0], A.y, fma(mz[0], A.z, A.w));
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) r2[q] = fma(mx[q], A.x, yz) + p2[q];
+    } else {
+#pragma unroll
+        for (int q = 0; q < PPL; ++q)
+            r2[q] = fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
+    }
+}
+
 // Energies (or in-cutoff counts) of this lane's PPL points against the LDS
 // list.  Points and records are relative to the tile centre c.
 //   Far pairs (inside r_c and beyond the FP64 switch radius for every point
@@ -436,7 +465,9 @@ __device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict
 //   Shell pairs: the same, behind the strict FP64 test r^2 < r_c^2.
 //   Near pairs: FP64 displacement and r^2, the reference's clamp and strict
 //   cutoff test, FP64 LJ with a Newton-refined reciprocal.
-template <int PPL, int MATH>
+// XL: see pair_r2 (never used for FP64 grids, whose near pairs use
+// displacements).
+template <int PPL, int MATH, bool XL>
 __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                                           const double4 *__restrict__ L, int cap,
                                           const int *__restrict__ seg, int nseg,
@@ -465,12 +496,10 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             // record is a shell record: the cutoff is sigma_max)
             const double sig2 = P.sig2;
             for (int k = sb; k < se; ++k) {
-                const double4 A = Ls[-k];
+                double r2[PPL];
+                pair_r2<PPL, XL>(Ls[-k], mx, my, mz, p2, r2);
 #pragma unroll
-                for (int q = 0; q < PPL; ++q) {
-                    const double r2 = fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
-                    acc[q] += r2 < sig2 ? 1.0 : 0.0;
-                }
+                for (int q = 0; q < PPL; ++q) acc[q] += r2[q] < sig2 ? 1.0 : 0.0;
             }
             continue;
         }
@@ -478,18 +507,39 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
 #pragma unroll
             for (int q = 0; q < PPL; ++q) acc[q] += (double)(fe - fb);
             for (int k = sb; k < se; ++k) {
-                const double4 A = Ls[-k];
+                double r2[PPL];
+                pair_r2<PPL, XL>(Ls[-k], mx, my, mz, p2, r2);
 #pragma unroll
-                for (int q = 0; q < PPL; ++q) {
-                    const double r2 = fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
-                    acc[q] += r2 < rc2 ? 1.0 : 0.0;
-                }
+                for (int q = 0; q < PPL; ++q) acc[q] += r2[q] < rc2 ? 1.0 : 0.0;
             }
             continue;
         }
         if (MATH == MATH_MIXED) {
             const double sig6 = P.sig2 * P.sig2 * P.sig2;
             const double c6 = P.eps4 * sig6 * sig6, c3 = -P.eps4 * sig6;
+            auto far_pair = [&](const double4 &A, float (&s3)[PPL], float (&s6)[PPL]) {
+                double r2[PPL];
+                pair_r2<PPL, XL>(A, mx, my, mz, p2, r2);
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) {
+                    const float w = __builtin_amdgcn_rcpf((float)r2[q]);
+                    const float w2 = w * w, w3 = w2 * w;
+                    s3[q] = __builtin_fmaf(w2, w, s3[q]);
+                    s6[q] = __builtin_fmaf(w3, w3, s6[q]);
+                }
+            };
+            auto shell_pair = [&](const double4 &A, float (&s3)[PPL], float (&s6)[PPL]) {
+                double r2[PPL];
+                pair_r2<PPL, XL>(A, mx, my, mz, p2, r2);
+#pragma unroll
+                for (int q = 0; q < PPL; ++q) {
+                    const float w = __builtin_amdgcn_rcpf((float)r2[q]);
+                    float w3 = w * w * w;
+                    w3 = r2[q] < rc2 ? w3 : 0.0f;  // src/computegridsheet.h:79
+                    s3[q] += w3;
+                    s6[q] = __builtin_fmaf(w3, w3, s6[q]);
+                }
+            };
             // far records [fb, fe), then shell records [sb, se), 8 pairs per flush
             for (int r = fb; r < fe; r += 8) {
                 float a3[PPL], a6[PPL];
@@ -497,89 +547,47 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                 for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
                 if (r + 8 <= fe) {
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const double4 A = L[r + u];
-#pragma unroll
-                        for (int q = 0; q < PPL; ++q) {
-                            const double r2 =
-                                fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
-                            const float w = __builtin_amdgcn_rcpf((float)r2);
-                            const float w3 = w * w * w;
-                            a3[q] += w3;
-                            a6[q] = __builtin_fmaf(w3, w3, a6[q]);
-                        }
-                    }
+                    for (int u = 0; u < 8; ++u) far_pair(L[r + u], a3, a6);
                 } else {
-                    for (int k = r; k < fe; ++k) {
-                        const double4 A = L[k];
-#pragma unroll
-                        for (int q = 0; q < PPL; ++q) {
-                            const double r2 =
-                                fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
-                            const float w = __builtin_amdgcn_rcpf((float)r2);
-                            const float w3 = w * w * w;
-                            a3[q] += w3;
-                            a6[q] = __builtin_fmaf(w3, w3, a6[q]);
-                        }
-                    }
+                    for (int k = r; k < fe; ++k) far_pair(L[k], a3, a6);
                 }
 #pragma unroll
                 for (int q = 0; q < PPL; ++q)
                     acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
             }
-            auto shell_pair = [&](const double4 &A, int q, float &s3, float &s6) {
-                const double r2 = fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q];
-                const float w = __builtin_amdgcn_rcpf((float)r2);
-                float w3 = w * w * w;
-                w3 = r2 < rc2 ? w3 : 0.0f;  // src/computegridsheet.h:79
-                s3 += w3;
-                s6 = __builtin_fmaf(w3, w3, s6);
-            };
             for (int k0 = sb; k0 < se; k0 += 8) {
                 float a3[PPL], a6[PPL];
 #pragma unroll
                 for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
                 if (k0 + 8 <= se) {
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) {
-                        const double4 A = Ls[-(k0 + u)];
-#pragma unroll
-                        for (int q = 0; q < PPL; ++q) shell_pair(A, q, a3[q], a6[q]);
-                    }
+                    for (int u = 0; u < 8; ++u) shell_pair(Ls[-(k0 + u)], a3, a6);
                 } else {
-                    for (int k = k0; k < se; ++k) {
-                        const double4 A = Ls[-k];
-#pragma unroll
-                        for (int q = 0; q < PPL; ++q) shell_pair(A, q, a3[q], a6[q]);
-                    }
+                    for (int k = k0; k < se; ++k) shell_pair(Ls[-k], a3, a6);
                 }
 #pragma unroll
                 for (int q = 0; q < PPL; ++q)
                     acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
             }
-        }
-        if (MATH == MATH_MIXED) {
             // Near pairs of an FP32 grid or Widom run: the expanded FP64 r^2 of
             // the far pairs (|a| <= r_FP64 + R keeps it exact to ~1e-14 A^2),
             // the reference's clamp and strict cutoff, u = 1/r^2 from the FP32
             // reciprocal and one Newton step (~1e-14 relative), and FP64 sums
             // of u^3 and u^6 scaled by 4 eps sigma^6 / sigma^12 per segment.
-            const double sig6 = P.sig2 * P.sig2 * P.sig2;
-            const double c6 = P.eps4 * sig6 * sig6, c3 = -P.eps4 * sig6;
             double b3[PPL], b6[PPL];
 #pragma unroll
             for (int q = 0; q < PPL; ++q) b3[q] = b6[q] = 0.0;
             for (int k = nb; k < ne; ++k) {
-                const double4 A = Ln[k];
+                double r2[PPL];
+                pair_r2<PPL, XL>(Ln[k], mx, my, mz, p2, r2);
 #pragma unroll
                 for (int q = 0; q < PPL; ++q) {
                     // src/computegridsheet.h:77 (r^2 is never NaN, so fmax is the clamp)
-                    const double r2 =
-                        fmax(fma(mx[q], A.x, fma(my[q], A.y, fma(mz[q], A.z, A.w))) + p2[q], min_r2);
-                    double y = (double)__builtin_amdgcn_rcpf((float)r2);
-                    y = fma(fma(-r2, y, 1.0), y, y);
+                    const double rr = fmax(r2[q], min_r2);
+                    double y = (double)__builtin_amdgcn_rcpf((float)rr);
+                    y = fma(fma(-rr, y, 1.0), y, y);
                     double y3 = y * y * y;
-                    y3 = r2 < rc2 ? y3 : 0.0;  // src/computegridsheet.h:79
+                    y3 = rr < rc2 ? y3 : 0.0;  // src/computegridsheet.h:79
                     b3[q] += y3;
                     b6[q] = fma(y3, y3, b6[q]);
                 }
@@ -595,7 +603,7 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
 #pragma unroll
             for (int q = 0; q < PPL; ++q) {
                 const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
-                const double r2 = fmax(dx * dx + dy * dy + dz * dz, min_r2);  // :77
+                const double r2 = fmax(fma(dx, dx, fma(dy, dy, dz * dz)), min_r2);  // :77
                 const bool in = r2 < rc2;  // src/computegridsheet.h:79
                 double y = (double)__builtin_amdgcn_rcpf((float)r2);
                 y = fma(fma(-r2, y, 1.0), y, y);
@@ -653,12 +661,18 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
         oidx[q] = (long long)gk + (long long)gj * J->n[2] + (long long)gi * J->n[1] * J->n[2];
         acc[q] = 0.0;
     }
+    // A lane's points are p = tid + q * 256: when t1 * t2 divides 256 they
+    // differ only in i, and when the cell's first vector lies along x
+    // (t_yx = t_zx = 0, as cell parameters give it) only in x.
+    const bool xl = MATH != MATH_FP64 && PPL > 1 && kWG % (t1 * t2) == 0 && J->t[3] == 0.0 &&
+                    J->t[6] == 0.0;
     int cur_t = 0, cur_pos = uniform(tbeg[0]);
     do {
         const int nseg =
             fill_list<MATH>(J, src, tbeg, tl, cur_t, cur_pos, cap, s_list, s_seg, s_cnt);
         __syncthreads();
-        eval_list<PPL, MATH>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
+        if (xl) eval_list<PPL, MATH, true>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
+        else eval_list<PPL, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
         __syncthreads();
     } while (cur_t < J->ntypes);
 #pragma unroll
@@ -826,6 +840,8 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             }
         } else {
             // ---- one Widom bin: every insertion whose point falls in it ----
+            // (at most 2 per lane: bins hold a few hundred insertions)
+            constexpr int PW = PPL > 2 ? 2 : PPL;
             const int nb0 = J->nbins[0], nb1 = J->nbins[1], nb2 = J->nbins[2];
             const int b = (int)local;
             const int bc[3] = {b / (nb1 * nb2), (b / nb2) % nb1, b % nb2};
@@ -845,12 +861,12 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                 const int nseg = fill_list<MATH>(J, J->near, J->type_begin, tl, cur_t, cur_pos,
                                                  cap, s_list, s_seg, s_cnt);
                 __syncthreads();
-                for (uint32_t c = s0; c < s1; c += (uint32_t)(PPL * kWG)) {
-                    double px[PPL], py[PPL], pz[PPL], acc[PPL];
-                    uint32_t ins[PPL];
-                    bool ok[PPL];
+                for (uint32_t c = s0; c < s1; c += (uint32_t)(PW * kWG)) {
+                    double px[PW], py[PW], pz[PW], acc[PW];
+                    uint32_t ins[PW];
+                    bool ok[PW];
 #pragma unroll
-                    for (int q = 0; q < PPL; ++q) {
+                    for (int q = 0; q < PW; ++q) {
                         const uint32_t p = c + (uint32_t)(q * kWG + tid);
                         ok[q] = p < s1;
                         ins[q] = J->sorted[ok[q] ? p : s0];
@@ -862,9 +878,9 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                         pz[q] -= tl.cz;
                         acc[q] = 0.0;
                     }
-                    eval_list<PPL, MATH>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
+                    eval_list<PW, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
 #pragma unroll
-                    for (int q = 0; q < PPL; ++q) {
+                    for (int q = 0; q < PW; ++q) {
                         if (!ok[q]) continue;
                         double *dst = J->e_out + ins[q];
                         *dst = first ? acc[q] : *dst + acc[q];
@@ -1040,10 +1056,10 @@ hipError_t launch_tiles(int math, int ppl, const JobDev *jobs, int njobs, long l
 #define ER_TILES(P, M)                                                                     \
     if (ppl == P && math == M)                                                             \
         return launch_tiles_t<P, M>(jobs, njobs, total, ctr, wgs, cap, scratch, scratch_cap, s);
-    ER_TILES(1, MATH_MIXED) ER_TILES(2, MATH_MIXED)
-    ER_TILES(1, MATH_FP64) ER_TILES(2, MATH_FP64)
-    ER_TILES(1, MATH_COUNT) ER_TILES(2, MATH_COUNT)
-    ER_TILES(1, MATH_OVERLAP) ER_TILES(2, MATH_OVERLAP)
+    ER_TILES(1, MATH_MIXED) ER_TILES(2, MATH_MIXED) ER_TILES(4, MATH_MIXED)
+    ER_TILES(1, MATH_FP64) ER_TILES(2, MATH_FP64) ER_TILES(4, MATH_FP64)
+    ER_TILES(1, MATH_COUNT) ER_TILES(2, MATH_COUNT) ER_TILES(4, MATH_COUNT)
+    ER_TILES(1, MATH_OVERLAP) ER_TILES(2, MATH_OVERLAP) ER_TILES(4, MATH_OVERLAP)
 #undef ER_TILES
     return hipErrorInvalidValue;
 }
@@ -1051,10 +1067,10 @@ hipError_t launch_tiles(int math, int ppl, const JobDev *jobs, int njobs, long l
 int tiles_occupancy(int math, int ppl, int cap) {
 #define ER_OCC(P, M) \
     if (ppl == P && math == M) return occupancy_t<P, M>(cap);
-    ER_OCC(1, MATH_MIXED) ER_OCC(2, MATH_MIXED)
-    ER_OCC(1, MATH_FP64) ER_OCC(2, MATH_FP64)
-    ER_OCC(1, MATH_COUNT) ER_OCC(2, MATH_COUNT)
-    ER_OCC(1, MATH_OVERLAP) ER_OCC(2, MATH_OVERLAP)
+    ER_OCC(1, MATH_MIXED) ER_OCC(2, MATH_MIXED) ER_OCC(4, MATH_MIXED)
+    ER_OCC(1, MATH_FP64) ER_OCC(2, MATH_FP64) ER_OCC(4, MATH_FP64)
+    ER_OCC(1, MATH_COUNT) ER_OCC(2, MATH_COUNT) ER_OCC(4, MATH_COUNT)
+    ER_OCC(1, MATH_OVERLAP) ER_OCC(2, MATH_OVERLAP) ER_OCC(4, MATH_OVERLAP)
 #undef ER_OCC
     return 1;
 }

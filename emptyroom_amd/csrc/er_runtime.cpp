@@ -387,12 +387,26 @@ int framework_upload(Context *ctx, const Particle *atoms, int n, const double t[
 // Job construction
 // ---------------------------------------------------------------------------
 
+int grid_size_class(const double t[9], const int n[3], const Tuning &tu) {
+    double lg = 0.0;
+    for (int d = 0; d < 3; d++) {
+        const double a = std::sqrt(t[d] * t[d] + t[3 + d] * t[3 + d] + t[6 + d] * t[6 + d]);
+        lg += std::log(a / std::max(n[d] - 1, 1));
+    }
+    const double h = std::exp(lg / 3.0);
+    if (h < tu.class_spacing[0]) return 0;
+    if (h < tu.class_spacing[1]) return 1;
+    return 2;
+}
+
+void grid_tile_shape(const double t[9], const int n[3], const Tuning &tu, int tile[3]) {
+    const int *ct = tu.class_tile[grid_size_class(t, n, tu)];
+    const bool set = ct[0] > 0 && ct[1] > 0 && ct[2] > 0;
+    for (int d = 0; d < 3; d++) tile[d] = set ? ct[d] : tu.tile[d];
+}
+
 void choose_grid_tile(const FrameworkDev *fw, const int n[3], const Tuning &tu, int tile[3]) {
-    (void)fw;
-    (void)n;
-    tile[0] = tu.tile[0];
-    tile[1] = tu.tile[1];
-    tile[2] = tu.tile[2];
+    grid_tile_shape(fw->t, n, tu, tile);
 }
 
 // Bins of near-cubic shape (per perpendicular width) whose mean occupancy is
@@ -552,22 +566,6 @@ int check_grid_dims(const int n[3]) {
     return ST_OK;
 }
 
-int check_tuning(const Tuning &tu) {
-    if (tu.ppl != 1 && tu.ppl != 2) {
-        set_error("tuning: points per lane must be 1 or 2");
-        return ST_ARG;
-    }
-    if (tu.tile[0] * tu.tile[1] * tu.tile[2] != kWG * tu.ppl || tu.tile[0] < 1 || tu.tile[1] < 1 ||
-        tu.tile[2] < 1) {
-        set_error("tuning: tile extents must multiply to 256 * points-per-lane");
-        return ST_ARG;
-    }
-    if (tu.cap < 2 * kWG || tu.cap > 4800) {
-        set_error("tuning: LDS list capacity must be in [512, 4800] records");
-        return ST_ARG;
-    }
-    return ST_OK;
-}
 
 int persistent_wgs(Context *ctx, int math, long long items) {
     return tiles_workgroups(ctx, math, items, ctx->scratch_wgs);
@@ -593,6 +591,36 @@ struct Scoped {
 };
 
 }  // namespace
+
+int check_tuning(const Tuning &tu) {
+    if (tu.ppl != 1 && tu.ppl != 2 && tu.ppl != 4) {
+        set_error("tuning: points per lane must be 1, 2 or 4");
+        return ST_ARG;
+    }
+    if (tu.tile[0] * tu.tile[1] * tu.tile[2] != kWG * tu.ppl || tu.tile[0] < 1 || tu.tile[1] < 1 ||
+        tu.tile[2] < 1) {
+        set_error("tuning: tile extents must multiply to 256 * points-per-lane");
+        return ST_ARG;
+    }
+    for (int c = 0; c < 3; c++) {
+        const int *ct = tu.class_tile[c];
+        if (ct[0] == 0 && ct[1] == 0 && ct[2] == 0) continue;
+        if (ct[0] < 1 || ct[1] < 1 || ct[2] < 1 || ct[0] * ct[1] * ct[2] != kWG * tu.ppl) {
+            set_error("tuning: a class tile must be all zero or multiply to 256 * points-per-lane");
+            return ST_ARG;
+        }
+    }
+    if (!(tu.class_spacing[0] > 0.0) || !(tu.class_spacing[1] >= tu.class_spacing[0]) ||
+        !(tu.class_spacing[1] < 1e6)) {
+        set_error("tuning: class spacings must satisfy 0 < h0 <= h1");
+        return ST_ARG;
+    }
+    if (tu.cap < 2 * kWG || tu.cap > 4800) {
+        set_error("tuning: LDS list capacity must be in [512, 4800] records");
+        return ST_ARG;
+    }
+    return ST_OK;
+}
 
 int widom_prepare(Context *ctx, FrameworkDev *fw, uint64_t n_ins, WidomBuffers *wb,
                   std::vector<void *> *owned) {

@@ -43,13 +43,25 @@ struct Particle {
 // of every FP64 sum, so a framework always runs with the same knobs whether it
 // runs alone or inside a batch.
 struct Tuning {
-    int ppl = 1;                 // points per lane (1 or 2)
+    int ppl = 2;                 // points per lane (1, 2 or 4)
     int cap = 1024;              // LDS list capacity (records of 32 B)
-    int tile[3] = {8, 8, 4};     // grid tile in points, product = 256 * ppl
+    int tile[3] = {8, 8, 8};     // grid tile in points, product = 256 * ppl
     int widom_per_bin = 480;     // largest mean insertions per Widom bin
     double s2_switch = 0.444;    // (sigma/r)^2 above which a pair takes the FP64 path (r < 1.5 sigma)
     double region_len = 4.0;     // edge of a region of grid tiles in Angstrom (0: no regions)
+    // grid tiles by size class (spacing thresholds in Angstrom; a zero row
+    // means `tile`)
+    double class_spacing[2] = {0.15, 0.35};
+    int class_tile[3][3] = {{0, 0, 0}, {0, 0, 0}, {0, 0, 0}};
 };
+
+// Size class (0..2) of a grid of n points on cell t: by the geometric mean
+// of the lattice-direction point spacings.
+int grid_size_class(const double t[9], const int n[3], const Tuning &tu);
+// Tile of such a grid: its class's row, or the default tile.
+void grid_tile_shape(const double t[9], const int n[3], const Tuning &tu, int tile[3]);
+// ST_OK, or ST_ARG with the error set, for knobs the kernels cannot run with.
+int check_tuning(const Tuning &tu);
 
 class Context;
 

@@ -74,7 +74,7 @@ typedef struct er_widom_result {
 
 /* Kernel shape knobs (see DESIGN notes in the repository). */
 typedef struct er_tuning {
-    int points_per_lane;   /* 1 or 2 */
+    int points_per_lane;   /* 1, 2 or 4 (Widom insertions: at most 2 per lane) */
     int lds_records;       /* LDS candidate-list capacity, 512..4800 records of 32 B */
     int tile[3];           /* grid tile in points, product = 256 * points_per_lane */
     int widom_per_bin;     /* largest mean insertions per Widom bin */
@@ -82,6 +82,15 @@ typedef struct er_tuning {
     double region_len;     /* edge in Angstrom of the tile regions that share one
                               candidate sub-list (0 = every tile scans the whole
                               near list) */
+    /* Grid tiles by size class.  A grid's spacing h is the geometric mean of
+     * its three lattice-direction point spacings |a_d| / (N_d - 1); class 0
+     * is h < class_spacing[0], class 1 is h < class_spacing[1], class 2 the
+     * rest.  A class whose class_tile row is all zero uses `tile`; a non-zero
+     * row must multiply to 256 * points_per_lane.  The class depends only on
+     * the cell and the point counts, so a grid gets the same tile alone, in a
+     * batch or in the server. */
+    double class_spacing[2];
+    int class_tile[3][3];
 } er_tuning;
 
 const char *er_last_error(void);
@@ -94,6 +103,14 @@ int er_context_create(int device, size_t hbm_cap_bytes, er_context **out);
 int er_context_destroy(er_context *ctx);
 int er_context_get_tuning(const er_context *ctx, er_tuning *out);
 int er_context_set_tuning(er_context *ctx, const er_tuning *tuning);
+/* Host only (no device): the library's default knobs. */
+int er_tuning_defaults(er_tuning *out);
+/* Host only (no device): the size class (0..2) and tile that `tuning` gives
+ * a grid of n points per axis on cell t_matrix (row-major, Cartesian =
+ * t_matrix * fractional).  No reference counterpart: the reference launches
+ * 16x16-point blocks per grid sheet (src/writegrid.cu:190-197). */
+int er_grid_tile_for(const er_tuning *tuning, const double t_matrix[9], const int n[3],
+                     int tile_out[3], int *size_class);
 int er_context_memory(const er_context *ctx, size_t *in_use, size_t *reserved);
 
 /* ---- frameworks ----------------------------------------------------------
