@@ -64,7 +64,14 @@ hipError_t launch_tiles(int math, int ppl, const JobDev *jobs_dev, int njobs,
                         long long total_items, unsigned long long *counter, int workgroups,
                         int cap, double4 *scratch, int scratch_cap, hipStream_t s);
 // LDS bytes the tiled kernel needs for a list capacity.
-size_t tiles_lds_bytes(int cap);
+size_t tiles_lds_bytes(int cap, int math);
+// Far + shell part of an FP32-grid LDS list of cap records (the rest holds
+// near records): both parts hold at least one chunk of kWG records
+// (cap >= 2 kWG), so a fill pass that starts empty always makes progress.
+__host__ __device__ inline int mixed_far_shell_cap(int cap) {
+    const int near_cap = (cap / 3) & ~63;
+    return cap - (near_cap < kWG ? kWG : near_cap);
+}
 // Resident workgroups per CU for (math, ppl, cap).
 int tiles_occupancy(int math, int ppl, int cap);
 

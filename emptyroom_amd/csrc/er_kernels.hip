@@ -318,10 +318,7 @@ template <int MATH>
 __device__ __forceinline__ int near_region_start(int cap) {
     if (MATH == MATH_FP64) return kWG;        // every record is near
     if (MATH == MATH_COUNT || MATH == MATH_OVERLAP) return cap - kWG;  // no near records
-    // both regions hold at least one chunk of kWG records (cap >= 2 kWG), so a
-    // fill pass that starts empty always makes progress
-    const int near_cap = (cap / 3) & ~63;
-    return cap - (near_cap < kWG ? kWG : near_cap);
+    return mixed_far_shell_cap(cap);
 }
 
 // Stream the near list of job J from (cur_t, cur_pos) into the LDS list until
@@ -467,12 +464,19 @@ __device__ __forceinline__ void pair_r2(const double4 &A, const double (&mx)[PPL
 //   cutoff test, FP64 LJ with a Newton-refined reciprocal.
 // XL: see pair_r2 (never used for FP64 grids, whose near pairs use
 // displacements).
+// ws (MIXED only, wave-uniform): this wave's points lie in the sphere wsph
+// (centre relative to the tile centre, radius w), smaller than the tile's;
+// each segment's shell records are then re-classified against it into the
+// wave's own lists of LDS byte offsets wl (see below).
 template <int PPL, int MATH, bool XL>
 __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                                           const double4 *__restrict__ L, int cap,
                                           const int *__restrict__ seg, int nseg,
                                           const double (&px)[PPL], const double (&py)[PPL],
-                                          const double (&pz)[PPL], double (&acc)[PPL]) {
+                                          const double (&pz)[PPL], double (&acc)[PPL],
+                                          bool ws = false,
+                                          double4 wsph = make_double4(0.0, 0.0, 0.0, 0.0),
+                                          uint16_t *__restrict__ wl = nullptr) {
     const double rc2 = J->rc2, min_r2 = J->min_r2;
     const int capA = near_region_start<MATH>(cap);
     const double4 *__restrict__ Ls = L + capA - 1;  // shell k at Ls[-k]
@@ -555,19 +559,84 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                 for (int q = 0; q < PPL; ++q)
                     acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
             }
-            for (int k0 = sb; k0 < se; k0 += 8) {
-                float a3[PPL], a6[PPL];
-#pragma unroll
-                for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
-                if (k0 + 8 <= se) {
-#pragma unroll
-                    for (int u = 0; u < 8; ++u) shell_pair(Ls[-(k0 + u)], a3, a6);
-                } else {
-                    for (int k = k0; k < se; ++k) shell_pair(Ls[-k], a3, a6);
+            if (ws) {
+                // Shell records beyond r_c + w of the wave's sphere are out of
+                // the cutoff for all of the wave's points, and those within
+                // r_c - w are inside it for all of them (no cutoff test); the
+                // wave sorts the segment's shell records into its own list of
+                // LDS byte offsets, order kept: wave-far ones from the front,
+                // still-shell ones from the back.
+                const int lane = threadIdx.x & 63;
+                const double rin = J->rc + wsph.w + kMargin, rfar = J->rc - wsph.w - kMargin;
+                const double in2 = rin * rin, far2 = rfar > 0.0 ? rfar * rfar : -1.0;
+                int nwf = 0, nws = 0;
+                for (int k0 = sb; k0 < se; k0 += 64) {
+                    const int k = k0 + lane;
+                    int cat = 0;
+                    if (k < se) {
+                        const double4 A = Ls[-k];
+                        const double dx = A.x - wsph.x, dy = A.y - wsph.y, dz = A.z - wsph.z;
+                        const double d2 = dx * dx + dy * dy + dz * dz;
+                        cat = d2 < in2 ? (d2 < far2 ? 1 : 2) : 0;
+                    }
+                    const uint64_t bf = __ballot(cat == 1), bs = __ballot(cat == 2);
+                    const uint16_t off = (uint16_t)((capA - 1 - k) * (int)sizeof(double4));
+                    if (cat == 1) wl[nwf + lanes_below(bf)] = off;
+                    if (cat == 2) wl[capA - 1 - (nws + lanes_below(bs))] = off;
+                    nwf = uniform(nwf + __popcll(bf));
+                    nws = uniform(nws + __popcll(bs));
                 }
+                // the lists are read back by every lane of this wave
+                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+                __builtin_amdgcn_wave_barrier();
+                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                const char *__restrict__ Lb = reinterpret_cast<const char *>(L);
+                auto rec = [&](int e) {
+                    return *reinterpret_cast<const double4 *>(Lb + wl[e]);
+                };
+                for (int e0 = 0; e0 < nwf; e0 += 8) {
+                    float a3[PPL], a6[PPL];
 #pragma unroll
-                for (int q = 0; q < PPL; ++q)
-                    acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
+                    for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
+                    if (e0 + 8 <= nwf) {
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) far_pair(rec(e0 + u), a3, a6);
+                    } else {
+                        for (int e = e0; e < nwf; ++e) far_pair(rec(e), a3, a6);
+                    }
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q)
+                        acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
+                }
+                for (int e0 = 0; e0 < nws; e0 += 8) {
+                    float a3[PPL], a6[PPL];
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
+                    if (e0 + 8 <= nws) {
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) shell_pair(rec(capA - 1 - (e0 + u)), a3, a6);
+                    } else {
+                        for (int e = e0; e < nws; ++e) shell_pair(rec(capA - 1 - e), a3, a6);
+                    }
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q)
+                        acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
+                }
+            } else {
+                for (int k0 = sb; k0 < se; k0 += 8) {
+                    float a3[PPL], a6[PPL];
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
+                    if (k0 + 8 <= se) {
+#pragma unroll
+                        for (int u = 0; u < 8; ++u) shell_pair(Ls[-(k0 + u)], a3, a6);
+                    } else {
+                        for (int k = k0; k < se; ++k) shell_pair(Ls[-k], a3, a6);
+                    }
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q)
+                        acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
+                }
             }
             // Near pairs of an FP32 grid or Widom run: the expanded FP64 r^2 of
             // the far pairs (|a| <= r_FP64 + R keeps it exact to ~1e-14 A^2),
@@ -623,7 +692,7 @@ template <int PPL, int MATH>
 __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
                           const double4 *__restrict__ src, const int *__restrict__ tbeg, int cap,
                           double4 *__restrict__ s_list, int *__restrict__ s_seg,
-                          int *__restrict__ s_cnt) {
+                          int *__restrict__ s_cnt, uint16_t *__restrict__ s_wl) {
     const int tid = threadIdx.x;
     int i0[3], i1[3];
     double lo[3], hi[3];
@@ -640,13 +709,29 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
     double px[PPL], py[PPL], pz[PPL], acc[PPL];
     long long oidx[PPL];
     bool ok[PPL];
-    const int t1 = J->tile[1], t2 = J->tile[2];
+    const int t0 = J->tile[0], t1 = J->tile[1], t2 = J->tile[2];
+    // Quadrant layout (FP32 grids, even t1 and t2, t0 divisible by PPL): wave w
+    // owns the (j, k) quadrant (w & 1, w >> 1) of the tile and all of its i;
+    // lane l holds points i = il + q * t0 / PPL.  Otherwise point p = tid +
+    // 256 q of the tile in k-fastest order.
+    const int wave = tid >> 6, lane = tid & 63;
+    // (the wave lists hold 16-bit LDS byte offsets into the far/shell part)
+    const bool quad = MATH == MATH_MIXED && t1 % 2 == 0 && t2 % 2 == 0 && t0 % PPL == 0 &&
+                      mixed_far_shell_cap(cap) * (int)sizeof(double4) <= 65536;
+    const int h1 = t1 / 2, h2 = t2 / 2;
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
-        const int p = q * kWG + tid;
-        int gi = i0[0] + p / (t1 * t2);
-        int gj = i0[1] + (p / t2) % t1;
-        int gk = i0[2] + p % t2;
+        int gi, gj, gk;
+        if (quad) {
+            gi = i0[0] + lane / (h1 * h2) + q * (t0 / PPL);
+            gj = i0[1] + (wave & 1) * h1 + (lane / h2) % h1;
+            gk = i0[2] + (wave >> 1) * h2 + lane % h2;
+        } else {
+            const int p = q * kWG + tid;
+            gi = i0[0] + p / (t1 * t2);
+            gj = i0[1] + (p / t2) % t1;
+            gk = i0[2] + p % t2;
+        }
         ok[q] = gi <= i1[0] && gj <= i1[1] && gk <= i1[2];
         if (!ok[q]) {
             gi = i0[0];
@@ -661,18 +746,41 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
         oidx[q] = (long long)gk + (long long)gj * J->n[2] + (long long)gi * J->n[1] * J->n[2];
         acc[q] = 0.0;
     }
-    // A lane's points are p = tid + q * 256: when t1 * t2 divides 256 they
-    // differ only in i, and when the cell's first vector lies along x
-    // (t_yx = t_zx = 0, as cell parameters give it) only in x.
-    const bool xl = MATH != MATH_FP64 && PPL > 1 && kWG % (t1 * t2) == 0 && J->t[3] == 0.0 &&
-                    J->t[6] == 0.0;
+    // A lane's points differ only in i (quadrant layout, or p = tid + 256 q
+    // with t1 * t2 dividing 256), so on a cell whose first vector lies along
+    // x (t_yx = t_zx = 0, as cell parameters give it) only in x.
+    const bool xl = MATH != MATH_FP64 && PPL > 1 && (quad || kWG % (t1 * t2) == 0) &&
+                    J->t[3] == 0.0 && J->t[6] == 0.0;
+    // The sphere around this wave's valid points (quadrant layout): its
+    // index box is the tile's i range times the quadrant's j and k ranges.
+    bool ws = false, idle = false;
+    double4 wsph = make_double4(0.0, 0.0, 0.0, 0.0);
+    if (quad) {
+        const int jb = i0[1] + (wave & 1) * h1, kb = i0[2] + (wave >> 1) * h2;
+        const int je = jb + h1 - 1 < i1[1] ? jb + h1 - 1 : i1[1];
+        const int ke = kb + h2 - 1 < i1[2] ? kb + h2 - 1 : i1[2];
+        idle = jb > i1[1] || kb > i1[2];  // no point of this wave is on the grid
+        if (!idle) {
+            const double wlo[3] = {lo[0], 1.0 * jb / (J->n[1] - 1), 1.0 * kb / (J->n[2] - 1)};
+            const double whi[3] = {hi[0], 1.0 * je / (J->n[1] - 1), 1.0 * ke / (J->n[2] - 1)};
+            const Tile w = tile_sphere(J->t, wlo, whi);
+            wsph = make_double4(w.cx - tl.cx, w.cy - tl.cy, w.cz - tl.cz, w.R);
+            ws = true;
+        }
+    }
     int cur_t = 0, cur_pos = uniform(tbeg[0]);
     do {
         const int nseg =
             fill_list<MATH>(J, src, tbeg, tl, cur_t, cur_pos, cap, s_list, s_seg, s_cnt);
         __syncthreads();
-        if (xl) eval_list<PPL, MATH, true>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
-        else eval_list<PPL, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
+        if (!idle) {
+            if (xl)
+                eval_list<PPL, MATH, true>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, ws, wsph,
+                                           s_wl);
+            else
+                eval_list<PPL, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, ws,
+                                            wsph, s_wl);
+        }
         __syncthreads();
     } while (cur_t < J->ntypes);
 #pragma unroll
@@ -751,6 +859,10 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
     const int tid = threadIdx.x;
     double4 *__restrict__ slice =
         scratch ? scratch + (size_t)blockIdx.x * (size_t)scratch_cap : nullptr;
+    // per-wave shell offset lists after the LDS list (FP32 grids only)
+    uint16_t *__restrict__ s_wl = MATH == MATH_MIXED
+        ? reinterpret_cast<uint16_t *>(s_list + cap) + (tid >> 6) * mixed_far_shell_cap(cap)
+        : nullptr;
 
     for (;;) {
         if (tid == 0) s_item = (long long)atomicAdd(counter, 1ull);
@@ -836,7 +948,7 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             for (int ti = 0; ti < nt; ++ti) {
                 const int tc[3] = {t0[0] + ti / (tn[1] * tn[2]), t0[1] + (ti / tn[2]) % tn[1],
                                    t0[2] + ti % tn[2]};
-                grid_tile<PPL, MATH>(J, tc, src, tbeg, cap, s_list, s_seg, s_cnt);
+                grid_tile<PPL, MATH>(J, tc, src, tbeg, cap, s_list, s_seg, s_cnt, s_wl);
             }
         } else {
             // ---- one Widom bin: every insertion whose point falls in it ----
@@ -983,7 +1095,7 @@ int grid_blocks(long long n, int per_block, int cap_blocks) {
 template <int PPL, int MATH>
 hipError_t launch_tiles_t(const JobDev *jobs, int njobs, long long total, unsigned long long *ctr,
                           int wgs, int cap, double4 *scratch, int scratch_cap, hipStream_t s) {
-    hipLaunchKernelGGL((k_tiles<PPL, MATH>), dim3(wgs), dim3(kWG), tiles_lds_bytes(cap), s, jobs,
+    hipLaunchKernelGGL((k_tiles<PPL, MATH>), dim3(wgs), dim3(kWG), tiles_lds_bytes(cap, MATH), s, jobs,
                        njobs, total, ctr, cap, scratch, scratch_cap);
     return hipGetLastError();
 }
@@ -992,7 +1104,7 @@ template <int PPL, int MATH>
 int occupancy_t(int cap) {
     int n = 0;
     if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&n, k_tiles<PPL, MATH>, kWG,
-                                                     tiles_lds_bytes(cap)) != hipSuccess)
+                                                     tiles_lds_bytes(cap, MATH)) != hipSuccess)
         return 1;
     return n > 0 ? n : 1;
 }
@@ -1045,7 +1157,11 @@ hipError_t launch_bin_all(const BinJob *jobs, int njobs, long long total_blocks,
     return hipGetLastError();
 }
 
-size_t tiles_lds_bytes(int cap) { return (size_t)cap * sizeof(double4); }
+size_t tiles_lds_bytes(int cap, int math) {
+    size_t b = (size_t)cap * sizeof(double4);
+    if (math == MATH_MIXED) b += (size_t)(kWG / 64) * mixed_far_shell_cap(cap) * sizeof(uint16_t);
+    return (b + 15) & ~(size_t)15;
+}
 
 hipError_t launch_tiles(int math, int ppl, const JobDev *jobs, int njobs, long long total,
                         unsigned long long *ctr, int wgs, int cap, double4 *scratch,
