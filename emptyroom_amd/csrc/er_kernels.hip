@@ -373,7 +373,7 @@ __device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict
             if (cur_t < ntypes) cur_pos = uniform(tbeg[cur_t]);
             continue;
         }
-        if (nf + ns + kWG > capA || nn + kWG > capN || nseg >= kMaxSeg) break;
+        if (nseg >= kMaxSeg) break;
         const int idx = cur_pos + tid;
         int cat = CAT_NONE;
         double4 A = make_double4(0.0, 0.0, 0.0, 0.0);
@@ -410,14 +410,53 @@ __device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict
             ts += cs;
             tn += cn;
         }
-        if (cat == CAT_FAR) L[of + lanes_below(bf)] = A;
-        if (cat == CAT_SHELL) L[capA - 1 - (os + lanes_below(bs))] = A;
-        if (cat == CAT_NEAR) L[capA + on + lanes_below(bn)] = A;  // A: (a - c, |a - c|^2)
+        // A pass takes the kept records in source order up to the first one
+        // that does not fit (far + shell share [0, capA), near [capA, cap)),
+        // so its content depends only on the sequence of kept records, not on
+        // the source's chunking: a region sub-list and the near list give the
+        // same passes.  Records after that one may also be written (into
+        // slots past the final counts, never read in this pass).
+        const int fs_before = of + os + lanes_below(bf | bs);
+        const int n_before = on + lanes_below(bn);
+        const bool fits = (cat == CAT_FAR || cat == CAT_SHELL) ? fs_before < capA
+                          : cat == CAT_NEAR                    ? n_before < capN
+                                                               : true;
+        if (fits && cat == CAT_FAR) L[of + lanes_below(bf)] = A;
+        if (fits && cat == CAT_SHELL) L[capA - 1 - (os + lanes_below(bs))] = A;
+        if (fits && cat == CAT_NEAR) L[capA + on + lanes_below(bn)] = A;  // A: (a - c, |a - c|^2)
+        const uint64_t bx = __ballot(!fits);
+        if (lane == 0) {
+            // this wave's first record that does not fit, and the counts before it
+            int *x = s_cnt + 3 * (kWG / 64) + 4 * wave;
+            if (bx) {
+                const int l = __builtin_ctzll(bx);
+                const uint64_t below = (1ull << l) - 1ull;
+                x[0] = l;
+                x[1] = of + __popcll(bf & below);
+                x[2] = os + __popcll(bs & below);
+                x[3] = on + __popcll(bn & below);
+            } else {
+                x[0] = 64;
+            }
+        }
+        __syncthreads();
+        int stop = -1;
+#pragma unroll
+        for (int w = kWG / 64 - 1; w >= 0; --w)
+            if (s_cnt[3 * (kWG / 64) + 4 * w] < 64) stop = w;
+        if (stop >= 0) {
+            const int *x = s_cnt + 3 * (kWG / 64) + 4 * stop;
+            nf = uniform(x[1]);
+            ns = uniform(x[2]);
+            nn = uniform(x[3]);
+            cur_pos += 64 * stop + uniform(x[0]);
+            __syncthreads();  // x is read by every thread before it is reused
+            break;
+        }
         nf = uniform(nf + tf);
         ns = uniform(ns + ts);
         nn = uniform(nn + tn);
         cur_pos += kWG;
-        __syncthreads();
     }
     if (nf > f0 || ns > s0 || nn > n0) {
         if (tid == 0) {
@@ -852,7 +891,7 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                                                int scratch_cap) {
     extern __shared__ double4 s_list[];
     __shared__ int s_seg[kSegInts * kMaxSeg];
-    __shared__ int s_cnt[3 * (kWG / 64)];
+    __shared__ int s_cnt[7 * (kWG / 64)];  // fill: 3 counts + 4 stop fields per wave
     __shared__ int s_rbeg[kMaxTypes + 1];
     __shared__ double s_red[kWG / 64];
     __shared__ long long s_item;
@@ -879,99 +918,108 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
         const JobDev *__restrict__ J = jobs + jl;
         const long long local = item - J->item_begin + J->local0;
 
-        if (J->kind == JOB_GRID) {
-            // ---- one region of tiles; index k + j*nz + i*ny*nz ----
-            long long rem = local;
-            int rg[3], t0[3], tn[3];
-            rg[2] = (int)(rem % J->nregions[2]);
-            rem /= J->nregions[2];
-            rg[1] = (int)(rem % J->nregions[1]);
-            rg[0] = (int)(rem / J->nregions[1]);
+        // ---- one region of cells (grid tiles or Widom bins); index
+        //      k + j * nz + i * ny * nz over the job's regions ----
+        const bool grid = J->kind == JOB_GRID;
+        int ncell[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) ncell[d] = grid ? J->ntiles[d] : J->nbins[d];
+        long long rem = local;
+        int rg[3], t0[3], tn[3];
+        rg[2] = (int)(rem % J->nregions[2]);
+        rem /= J->nregions[2];
+        rg[1] = (int)(rem % J->nregions[1]);
+        rg[0] = (int)(rem / J->nregions[1]);
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            t0[d] = rg[d] * J->region[d];
+            const int e = t0[d] + J->region[d];
+            tn[d] = (e < ncell[d] ? e : ncell[d]) - t0[d];
+        }
+        const int nt = tn[0] * tn[1] * tn[2];
+        // fractional box of cell tc: a tile's grid points, or a bin
+        auto cell_box = [&](const int tc[3], double lo[3], double hi[3]) {
 #pragma unroll
             for (int d = 0; d < 3; ++d) {
-                t0[d] = rg[d] * J->region[d];
-                const int e = t0[d] + J->region[d];
-                tn[d] = (e < J->ntiles[d] ? e : J->ntiles[d]) - t0[d];
-            }
-            const int nt = tn[0] * tn[1] * tn[2];
-            const double4 *src = J->near;
-            const int *tbeg = J->type_begin;
-            if (nt > 1 && slice && J->ntypes <= kMaxTypes) {
-                // region centre: middle of the region's point box
-                double lo[3], hi[3];
-#pragma unroll
-                for (int d = 0; d < 3; ++d) {
-                    const int a = t0[d] * J->tile[d];
-                    const int b = (t0[d] + tn[d]) * J->tile[d];
+                if (grid) {
+                    const int a = tc[d] * J->tile[d];  // src/writegrid.cu:122-127
+                    const int b = a + J->tile[d];
                     lo[d] = 1.0 * a / (J->n[d] - 1);
                     hi[d] = 1.0 * ((b < J->n[d] ? b : J->n[d]) - 1) / (J->n[d] - 1);
-                }
-                double cx, cy, cz;
-                frac_to_cart(J->t, 0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]),
-                             0.5 * (lo[2] + hi[2]), cx, cy, cz);
-                // R = max over tiles of |c_t - c| + R_t (one tile per thread)
-                double r = 0.0;
-                for (int ti = tid; ti < nt; ti += kWG) {
-                    const int tc[3] = {t0[0] + ti / (tn[1] * tn[2]),
-                                       t0[1] + (ti / tn[2]) % tn[1], t0[2] + ti % tn[2]};
-                    double tlo[3], thi[3];
-#pragma unroll
-                    for (int d = 0; d < 3; ++d) {
-                        const int a = tc[d] * J->tile[d];
-                        const int b = a + J->tile[d];
-                        tlo[d] = 1.0 * a / (J->n[d] - 1);
-                        thi[d] = 1.0 * ((b < J->n[d] ? b : J->n[d]) - 1) / (J->n[d] - 1);
-                    }
-                    const Tile ts = tile_sphere(J->t, tlo, thi);
-                    const double dx = ts.cx - cx, dy = ts.cy - cy, dz = ts.cz - cz;
-                    const double rt = sqrt(dx * dx + dy * dy + dz * dz) + ts.R;
-                    r = rt > r ? rt : r;
-                }
-#pragma unroll
-                for (int o = 32; o >= 1; o >>= 1) {
-                    const double v = __shfl_xor(r, o, 64);
-                    r = v > r ? v : r;
-                }
-                if ((tid & 63) == 0) s_red[tid >> 6] = r;
-                __syncthreads();
-                double R = s_red[0];
-#pragma unroll
-                for (int w = 1; w < kWG / 64; ++w) R = s_red[w] > R ? s_red[w] : R;
-                R = R * (1.0 + 1e-12) + kMargin;
-                __syncthreads();
-                const double rr = J->rc + R + kMargin;
-                if (build_region_list(J, cx, cy, cz, rr * rr, slice, scratch_cap, s_rbeg, s_cnt)) {
-                    src = slice;
-                    tbeg = s_rbeg;
+                } else {
+                    lo[d] = (double)tc[d] / J->nbins[d];
+                    hi[d] = (double)(tc[d] + 1) / J->nbins[d];
                 }
             }
-            for (int ti = 0; ti < nt; ++ti) {
-                const int tc[3] = {t0[0] + ti / (tn[1] * tn[2]), t0[1] + (ti / tn[2]) % tn[1],
-                                   t0[2] + ti % tn[2]};
+        };
+        auto cell_of = [&](int ti, int tc[3]) {
+            tc[0] = t0[0] + ti / (tn[1] * tn[2]);
+            tc[1] = t0[1] + (ti / tn[2]) % tn[1];
+            tc[2] = t0[2] + ti % tn[2];
+        };
+        const double4 *src = J->near;
+        const int *tbeg = J->type_begin;
+        if (nt > 1 && slice && J->ntypes <= kMaxTypes) {
+            // region centre: middle of the region's box
+            double lo[3], hi[3], clo[3], chi[3];
+            int tc[3];
+            cell_of(0, tc);
+            cell_box(tc, lo, clo);
+            cell_of(nt - 1, tc);
+            cell_box(tc, chi, hi);
+            double cx, cy, cz;
+            frac_to_cart(J->t, 0.5 * (lo[0] + hi[0]), 0.5 * (lo[1] + hi[1]),
+                         0.5 * (lo[2] + hi[2]), cx, cy, cz);
+            // R = max over cells of |c_t - c| + R_t (one cell per thread)
+            double r = 0.0;
+            for (int ti = tid; ti < nt; ti += kWG) {
+                cell_of(ti, tc);
+                double tlo[3], thi[3];
+                cell_box(tc, tlo, thi);
+                const Tile ts = tile_sphere(J->t, tlo, thi);
+                const double dx = ts.cx - cx, dy = ts.cy - cy, dz = ts.cz - cz;
+                const double rt = sqrt(dx * dx + dy * dy + dz * dz) + ts.R;
+                r = rt > r ? rt : r;
+            }
+#pragma unroll
+            for (int o = 32; o >= 1; o >>= 1) {
+                const double v = __shfl_xor(r, o, 64);
+                r = v > r ? v : r;
+            }
+            if ((tid & 63) == 0) s_red[tid >> 6] = r;
+            __syncthreads();
+            double R = s_red[0];
+#pragma unroll
+            for (int w = 1; w < kWG / 64; ++w) R = s_red[w] > R ? s_red[w] : R;
+            R = R * (1.0 + 1e-12) + kMargin;
+            __syncthreads();
+            const double rr = J->rc + R + kMargin;
+            if (build_region_list(J, cx, cy, cz, rr * rr, slice, scratch_cap, s_rbeg, s_cnt)) {
+                src = slice;
+                tbeg = s_rbeg;
+            }
+        }
+        for (int ti = 0; ti < nt; ++ti) {
+            int tc[3];
+            cell_of(ti, tc);
+            if (grid) {
                 grid_tile<PPL, MATH>(J, tc, src, tbeg, cap, s_list, s_seg, s_cnt, s_wl);
+                continue;
             }
-        } else {
             // ---- one Widom bin: every insertion whose point falls in it ----
             // (at most 2 per lane: bins hold a few hundred insertions)
             constexpr int PW = PPL > 2 ? 2 : PPL;
-            const int nb0 = J->nbins[0], nb1 = J->nbins[1], nb2 = J->nbins[2];
-            const int b = (int)local;
-            const int bc[3] = {b / (nb1 * nb2), (b / nb2) % nb1, b % nb2};
-            const int nbv[3] = {nb0, nb1, nb2};
-            double lo[3], hi[3];
-#pragma unroll
-            for (int d = 0; d < 3; ++d) {
-                lo[d] = (double)bc[d] / nbv[d];
-                hi[d] = (double)(bc[d] + 1) / nbv[d];
-            }
+            const int b = (tc[0] * J->nbins[1] + tc[1]) * J->nbins[2] + tc[2];
             const uint32_t s0 = J->bin_start[b], s1 = J->bin_start[b + 1];
             if (s0 == s1) continue;
+            double lo[3], hi[3];
+            cell_box(tc, lo, hi);
             const Tile tl = tile_sphere(J->t, lo, hi);
-            int cur_t = 0, cur_pos = uniform(J->type_begin[0]);
+            int cur_t = 0, cur_pos = uniform(tbeg[0]);
             bool first = true;
             do {
-                const int nseg = fill_list<MATH>(J, J->near, J->type_begin, tl, cur_t, cur_pos,
-                                                 cap, s_list, s_seg, s_cnt);
+                const int nseg = fill_list<MATH>(J, src, tbeg, tl, cur_t, cur_pos, cap, s_list,
+                                                 s_seg, s_cnt);
                 __syncthreads();
                 for (uint32_t c = s0; c < s1; c += (uint32_t)(PW * kWG)) {
                     double px[PW], py[PW], pz[PW], acc[PW];

@@ -443,20 +443,21 @@ void choose_widom_bins(const FrameworkDev *fw, uint64_t n_ins, const Tuning &tu,
     }
 }
 
-// Tiles per region per axis: about region_len Angstrom along each lattice
-// vector, shrunk (largest extent first) until the expected sub-list -- the
-// framework's atom density times the sphere of radius r_c + region radius --
-// fills at most 3/4 of a workgroup's scratch and the grid has at least
-// kMinGridItems regions (a small grid alone must still fill the device).
-// Regions of one tile skip the sub-list.  Depends only on the framework, the
-// grid and the tuning.
-void choose_grid_region(const FrameworkDev *fw, const int n[3], const int tile[3],
-                        const Tuning &tu, int region[3]) {
+// Cells (grid tiles or Widom bins) per region per axis: about region_len
+// Angstrom along each lattice vector, shrunk (largest extent first) until the
+// expected sub-list -- the framework's atom density times the sphere of
+// radius r_c + region radius -- fills at most 3/4 of a workgroup's scratch
+// and the job has at least min_items regions (so a job launched alone still
+// fills the device).  cell_frac[d] is a cell's fractional extent, ncell[d]
+// the cells per axis.  Regions of one cell skip the sub-list.  A cell's
+// candidate list, and so every result, is the same whatever region it is in.
+void choose_region(const FrameworkDev *fw, const double cell_frac[3], const int ncell[3],
+                   const Tuning &tu, long long min_items, int region[3]) {
     double edge[3];
     for (int d = 0; d < 3; d++) {
         const double a = std::sqrt(fw->t[d] * fw->t[d] + fw->t[3 + d] * fw->t[3 + d] +
                                    fw->t[6 + d] * fw->t[6 + d]);
-        edge[d] = a * tile[d] / (n[d] - 1);
+        edge[d] = a * cell_frac[d];
         region[d] = 1;
         if (tu.region_len > 0.0 && edge[d] > 0.0)
             region[d] = std::max(1, (int)std::lround(tu.region_len / edge[d]));
@@ -470,7 +471,7 @@ void choose_grid_region(const FrameworkDev *fw, const int n[3], const int tile[3
         for (int c = 0; c < 8; c++) {
             double f[3];
             for (int d = 0; d < 3; d++)
-                f[d] = ((c >> d) & 1 ? 0.5 : -0.5) * region[d] * tile[d] / (n[d] - 1);
+                f[d] = ((c >> d) & 1 ? 0.5 : -0.5) * region[d] * cell_frac[d];
             double x = 0, y = 0, z = 0;
             x = fw->t[0] * f[0] + fw->t[1] * f[1] + fw->t[2] * f[2];
             y = fw->t[3] * f[0] + fw->t[4] * f[1] + fw->t[5] * f[2];
@@ -479,19 +480,26 @@ void choose_grid_region(const FrameworkDev *fw, const int n[3], const int tile[3
         }
         const double rr = fw->rc + std::sqrt(r2) + 0.01;
         const double expect = dens * (4.0 / 3.0) * pi * rr * rr * rr;
-        // keep enough work items for a launch of this job alone to fill the device
         long long items = 1;
-        for (int d = 0; d < 3; d++) {
-            const long long nt = (n[d] + tile[d] - 1) / tile[d];
-            items *= (nt + region[d] - 1) / region[d];
-        }
-        if (expect <= 0.75 * kRegionRecords && items >= kMinGridItems) break;
+        for (int d = 0; d < 3; d++) items *= (ncell[d] + region[d] - 1) / region[d];
+        if (expect <= 0.75 * kRegionRecords && items >= min_items) break;
         int big = 0;
         for (int d = 1; d < 3; d++)
             if (region[d] * edge[d] > region[big] * edge[big]) big = d;
         if (region[big] == 1) break;
         region[big] = (region[big] + 1) / 2;
     }
+}
+
+void choose_grid_region(const FrameworkDev *fw, const int n[3], const int tile[3],
+                        const Tuning &tu, int region[3]) {
+    double frac[3];
+    int nt[3];
+    for (int d = 0; d < 3; d++) {
+        frac[d] = (double)tile[d] / (n[d] - 1);
+        nt[d] = (n[d] + tile[d] - 1) / tile[d];
+    }
+    choose_region(fw, frac, nt, tu, kMinGridItems, region);
 }
 
 JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void *d_out,
@@ -526,7 +534,7 @@ JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void
 }
 
 JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
-                      const WidomBuffers &wb) {
+                      const WidomBuffers &wb, const Tuning &tu, long long min_items) {
     JobDev j;
     std::memset(&j, 0, sizeof(j));
     std::memcpy(j.t, fw->t, sizeof(j.t));
@@ -545,7 +553,17 @@ JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
     j.seed = seed;
     j.stream = 0;
     j.e_out = wb.e;
-    j.n_items = (long long)wb.nbins[0] * wb.nbins[1] * wb.nbins[2];
+    // regions of bins share one candidate sub-list, like regions of grid tiles
+    double frac[3];
+    for (int d = 0; d < 3; d++) frac[d] = 1.0 / wb.nbins[d];
+    choose_region(fw, frac, wb.nbins, tu, min_items, j.region);
+    long long items = 1;
+    for (int d = 0; d < 3; d++) {
+        j.region[d] = std::min(j.region[d], wb.nbins[d]);
+        j.nregions[d] = (wb.nbins[d] + j.region[d] - 1) / j.region[d];
+        items *= j.nregions[d];
+    }
+    j.n_items = items;
     (void)n_ins;
     return j;
 }
@@ -760,7 +778,7 @@ int widom_run(Context *ctx, FrameworkDev *fw, uint64_t n_ins, double temperature
     Scoped tmp{ctx, {}};
     WidomBuffers wb;
     ST_TRY(widom_prepare(ctx, fw, n_ins, &wb, &tmp.ptrs));
-    JobDev job = make_widom_job(fw, n_ins, seed, wb);
+    JobDev job = make_widom_job(fw, n_ins, seed, wb, ctx->tuning, kMinGridItems);
     JobDev *d_job = nullptr;
     ST_TRY(pool_alloc(ctx, 1, &d_job, &tmp.ptrs));
     hipStream_t s = ctx->stream;
@@ -848,7 +866,7 @@ int count_incut_widom(Context *ctx, FrameworkDev *fw, uint64_t n_ins, uint64_t s
     JobDev *d_job = nullptr;
     ST_TRY(pool_alloc(ctx, 1, &d_sum, &tmp.ptrs));
     ST_TRY(pool_alloc(ctx, 1, &d_job, &tmp.ptrs));
-    JobDev job = make_widom_job(fw, n_ins, seed, wb);
+    JobDev job = make_widom_job(fw, n_ins, seed, wb, ctx->tuning, kMinGridItems);
     hipStream_t s = ctx->stream;
     HIP_TRY(hipMemcpyAsync(d_job, &job, sizeof(job), hipMemcpyHostToDevice, s));
     ST_TRY(widom_enqueue_binning(ctx, wb, n_ins, seed, 0u, s));
@@ -884,7 +902,7 @@ double job_cost_per_item(const FrameworkDev *fw, const JobDev &j, const Tuning &
     double pts;
     if (j.kind == JOB_GRID)
         pts = (double)j.tile[0] * j.tile[1] * j.tile[2] * j.region[0] * j.region[1] * j.region[2];
-    else pts = (double)tu.widom_per_bin;
+    else pts = (double)tu.widom_per_bin * j.region[0] * j.region[1] * j.region[2];
     return pts * dens * (4.0 / 3.0) * pi * fw->rc2 * fw->rc;
 }
 
@@ -994,7 +1012,10 @@ int batch_create(Context *ctx, const Task *tasks, int n, bool use_graph, Batch *
             b->bin_max = std::max<long long>(
                 b->bin_max, (long long)bj.nb[0] * bj.nb[1] * bj.nb[2]);
             b->bin_jobs.push_back(bj);
-            JobDev j = make_widom_job(tk.fw, tk.n_ins, tk.seed, b->wbufs[t]);
+            // many jobs share the launch: a Widom job needs only its share of
+            // the device's work items
+            JobDev j = make_widom_job(tk.fw, tk.n_ins, tk.seed, b->wbufs[t], ctx->tuning,
+                                      std::max<long long>(1, kMinGridItems / std::max(n, 1)));
             mixed.push_back({job_cost_per_item(tk.fw, j, ctx->tuning), j});
         }
     }

@@ -175,8 +175,12 @@ void choose_grid_region(const FrameworkDev *fw, const int n[3], const int tile[3
                         const Tuning &tu, int region[3]);
 JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void *d_out,
                      const Tuning &tu, int math);
+void choose_region(const FrameworkDev *fw, const double cell_frac[3], const int ncell[3],
+                   const Tuning &tu, long long min_items, int region[3]);
+// min_items: regions of bins are shrunk until the job has at least this many
+// work items (kMinGridItems for a job launched alone).
 JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
-                      const WidomBuffers &wb);
+                      const WidomBuffers &wb, const Tuning &tu, long long min_items);
 // Allocates the Widom buffers of a job (owned pointers appended to `owned`).
 int widom_prepare(Context *ctx, FrameworkDev *fw, uint64_t n_ins, WidomBuffers *wb,
                   std::vector<void *> *owned);

@@ -163,7 +163,8 @@ bool Server::admit_one(Req *r) {
     if (st == ST_OK && s.n_ins) {
         st = widom_prepare(ctx_, r->fw, s.n_ins, &r->wb, &r->owned);
         if (st == ST_OK) {
-            r->widom_job = make_widom_job(r->fw, s.n_ins, s.seed, r->wb);
+            r->widom_job = make_widom_job(r->fw, s.n_ins, s.seed, r->wb, ctx_->tuning,
+                                          kMinGridItems);
             r->items_total += r->widom_job.n_items;
             st = widom_enqueue_binning(ctx_, r->wb, s.n_ins, s.seed, 0u, ctx_->stream);
         }
@@ -310,9 +311,8 @@ void Server::run() {
                     long long &next = kind == 0 ? r->grid_next : r->widom_next;
                     const long long left = src.n_items - next;
                     if (left <= 0) continue;
-                    // a grid item is a region of tiles; a Widom item is one bin
-                    const long long w =
-                        kind == 0 ? (long long)src.region[0] * src.region[1] * src.region[2] : 1;
+                    // a grid item is a region of tiles, a Widom item a region of bins
+                    const long long w = (long long)src.region[0] * src.region[1] * src.region[2];
                     const long long take =
                         std::min(left, std::max<long long>(1, std::min(share, budget) / w));
                     JobDev j = src;

@@ -83,9 +83,11 @@ def test_regions_off_and_on_agree(rctx):
                        api.compute_grid(rctx, fw, n, api.ER_FP32))
         fw.close()
     set_region(rctx, 4.0)
-    g0, g1 = out[0.0][0], out[3.0][0]
-    assert (np.abs(g0 - g1) / np.maximum(1.0, np.abs(g0))).max() < 1e-12
-    check_fp32(out[3.0][1], g0)
+    # a fill pass takes kept records up to the first that does not fit, so a
+    # tile builds the same passes from its region's sub-list as from the near
+    # list: bit-identical grids
+    assert np.array_equal(out[0.0][0], out[3.0][0])
+    assert np.array_equal(out[0.0][1], out[3.0][1])
 
 
 def test_region_batch_and_server_bit_identical(rctx):
@@ -154,3 +156,34 @@ def test_region_overflow_falls_back(rctx):
     got = g64[idx[:, 0], idx[:, 1], idx[:, 2]]
     assert (np.abs(got - ref) / np.maximum(1.0, np.abs(ref))).max() < 1e-9
     check_fp32(g32[idx[:, 0], idx[:, 1], idx[:, 2]], ref)
+
+
+def test_widom_regions_batch_bit_identical_to_plain(rctx):
+    """Widom bins of a batch are grouped into regions that share one sub-list
+    (a batch of 32 jobs needs only 2048 / 32 items per job), while a plain
+    run keeps one bin per item; a bin's candidate list, and so every energy
+    and both sums, must be the same either way."""
+    print("widom regions batch vs plain", flush=True)
+    rctx.tuning = api.default_tuning()
+    fs = [synth.small_framework(300 + s, n_atoms=40 + 3 * s,
+                                cell=(11.0 + 0.2 * s, 12.0, 13.0, 85.0, 95.0, 100.0))
+          for s in range(32)]
+    fws = [upload(rctx, *(lambda a: (a[:, :3], a[:, 3], a[:, 4]))(f.particles()), f.t, f.rep,
+                  f.rc2) for f in fs]
+    tasks = [dict(fw=fw, grid_n=[0, 0, 0], precision=api.ER_FP32, n_insertions=300000,
+                  temperature=298.0, seed=40 + i) for i, fw in enumerate(fws)]
+    b = api.Batch(rctx, tasks, use_graph=False)
+    b.run()
+    b.wait()
+    for i, t in enumerate(tasks):
+        assert b.widom(i) == api.widom(rctx, t["fw"], t["n_insertions"], 298.0, t["seed"]), i
+    b.close()
+    # and the plain energies against the oracle on the first insertions
+    f, fw = fs[5], fws[5]
+    a = f.particles()
+    _, _, _, e = api.widom(rctx, fw, 300000, 298.0, 45, energies=True)
+    _, _, re = oracle.widom(f.t, a, f.rep, f.rc2, 298.0, 45, 0, 4000, energies=True)
+    mask = re < 1e4
+    assert np.max(np.abs(e[:4000][mask] - re[mask])) <= 1e-3
+    for fw in fws:
+        fw.close()
