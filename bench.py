@@ -77,6 +77,7 @@ def cpu_baseline(fws, uploaded, ctx, batch, budget_s, threads):
     grid_err = 0.0
     kh_err = 0.0
     e_err = 0.0
+    n_grid_cmp = n_kh_cmp = 0  # points with E < 1e4 K / runs with a nonzero sum compared
     for i in idx:
         f = fws[i]
         a = f.particles()
@@ -97,15 +98,24 @@ def cpu_baseline(fws, uploaded, ctx, batch, budget_s, threads):
         m = ref < 1e4
         if m.any():
             grid_err = max(grid_err, float(np.max(np.abs(g[m] - ref[m]))))
+            n_grid_cmp += int(m.sum())
         sb, seb, _ = api.widom(ctx, uploaded[i], nins, f.temperature, f.seed)
         if rb > 0:
             kh_err = max(kh_err, abs(sb - rb) / rb)
             e_err = max(e_err, abs(seb / sb - reb / rb))
+            n_kh_cmp += 1
         elif sb != 0:
             kh_err = float("inf")
+            n_kh_cmp += 1
+    # nothing comparable (every sampled point blocked): no error figure
+    if n_grid_cmp == 0:
+        grid_err = None
+    if n_kh_cmp == 0:
+        kh_err = e_err = None
     sample = (f"{len(idx)} of {len(fws)} frameworks, every k-th x-sheet of each grid and the "
               f"first insertions of each Widom run ({evals:.3g} pair-evals, FP64 oracle, "
-              f"{threads} threads)")
+              f"{threads} threads; errors over {n_grid_cmp} grid points with E < 1e4 K and "
+              f"{n_kh_cmp} Widom runs with a nonzero sum)")
     return dict(value=evals / secs, unit="pair-evals/s", cores=threads, kind="port",
                 sample=sample, seconds=round(secs, 2)), grid_err, kh_err, e_err
 
