@@ -18,6 +18,8 @@
 
 #include <float.h>
 
+#include <type_traits>
+
 // No implicit multiply-add contraction in this file: every FMA below is
 // written out, so the arithmetic of a pair (and so every energy) is fixed by
 // the source and does not depend on how the compiler schedules one template
@@ -513,7 +515,7 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                                           const int *__restrict__ seg, int nseg,
                                           const double (&px)[PPL], const double (&py)[PPL],
                                           const double (&pz)[PPL], double (&acc)[PPL],
-                                          bool ws = false,
+                                          double R, bool ws = false,
                                           double4 wsph = make_double4(0.0, 0.0, 0.0, 0.0),
                                           uint16_t *__restrict__ wl = nullptr) {
     const double rc2 = J->rc2, min_r2 = J->min_r2;
@@ -685,21 +687,30 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             double b3[PPL], b6[PPL];
 #pragma unroll
             for (int q = 0; q < PPL; ++q) b3[q] = b6[q] = 0.0;
-            for (int k = nb; k < ne; ++k) {
-                double r2[PPL];
-                pair_r2<PPL, XL>(Ln[k], mx, my, mz, p2, r2);
+            // A near record lies within r_FP64 + R of the tile centre, so
+            // within r_FP64 + 2R of every point: when that is inside r_c the
+            // strict cutoff test (src/computegridsheet.h:79) passes for every
+            // pair and is skipped.
+            const bool all_in = P.rclose + 2.0 * R + 3.0 * kMargin < J->rc;
+            auto near_pairs = [&](auto test) {
+                for (int k = nb; k < ne; ++k) {
+                    double r2[PPL];
+                    pair_r2<PPL, XL>(Ln[k], mx, my, mz, p2, r2);
 #pragma unroll
-                for (int q = 0; q < PPL; ++q) {
-                    // src/computegridsheet.h:77 (r^2 is never NaN, so fmax is the clamp)
-                    const double rr = fmax(r2[q], min_r2);
-                    double y = (double)__builtin_amdgcn_rcpf((float)rr);
-                    y = fma(fma(-rr, y, 1.0), y, y);
-                    double y3 = y * y * y;
-                    y3 = rr < rc2 ? y3 : 0.0;  // src/computegridsheet.h:79
-                    b3[q] += y3;
-                    b6[q] = fma(y3, y3, b6[q]);
+                    for (int q = 0; q < PPL; ++q) {
+                        // src/computegridsheet.h:77 (r^2 is never NaN, so fmax is the clamp)
+                        const double rr = fmax(r2[q], min_r2);
+                        double y = (double)__builtin_amdgcn_rcpf((float)rr);
+                        y = fma(fma(-rr, y, 1.0), y, y);
+                        double y3 = y * y * y;
+                        if (decltype(test)::value) y3 = rr < rc2 ? y3 : 0.0;  // :79
+                        b3[q] += y3;
+                        b6[q] = fma(y3, y3, b6[q]);
+                    }
                 }
-            }
+            };
+            if (all_in) near_pairs(std::integral_constant<bool, false>());
+            else near_pairs(std::integral_constant<bool, true>());
 #pragma unroll
             for (int q = 0; q < PPL; ++q) acc[q] = fma(c6, b6[q], fma(c3, b3[q], acc[q]));
             continue;
@@ -814,11 +825,11 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
         __syncthreads();
         if (!idle) {
             if (xl)
-                eval_list<PPL, MATH, true>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, ws, wsph,
-                                           s_wl);
+                eval_list<PPL, MATH, true>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R, ws,
+                                           wsph, s_wl);
             else
-                eval_list<PPL, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, ws,
-                                            wsph, s_wl);
+                eval_list<PPL, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
+                                            ws, wsph, s_wl);
         }
         __syncthreads();
     } while (cur_t < J->ntypes);
@@ -1038,7 +1049,7 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                         pz[q] -= tl.cz;
                         acc[q] = 0.0;
                     }
-                    eval_list<PW, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc);
+                    eval_list<PW, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R);
 #pragma unroll
                     for (int q = 0; q < PW; ++q) {
                         if (!ok[q]) continue;
