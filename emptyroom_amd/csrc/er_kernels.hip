@@ -8,12 +8,17 @@
 //                                  lie within r_c of the unit cell, grouped by type,
 //                                  reference enumeration order inside a type
 //   types   : TypeParams[ntypes]   mixed sigma^2, 4 eps, FP64 switch radius
-// The tiled kernel gives each workgroup one tile (a box of grid points or one
-// Widom bin).  The workgroup streams the near list once, keeps the image atoms
-// within r_c + R_tile of the tile in LDS (split into a "far" list that needs no
-// cutoff test and no FP64, and a "full" list), then every lane evaluates its
-// point(s) against the LDS list with the atom broadcast from LDS and the pair
-// parameters in scalar registers.
+// The tiled kernel (k_tiles) is persistent: each workgroup takes work items
+// (a region of grid tiles or of Widom bins) from a device counter, gathers the
+// region's candidate sub-list into its scratch slice, then per tile (a box of
+// grid points, or one bin's insertions) keeps the image atoms within
+// r_c + R_tile in LDS, split into far records (inside the cutoff and beyond
+// the FP64 switch radius for every point: no test, FP32 reciprocal powers),
+// shell records (cutoff test) and near records (FP64 pair arithmetic).
+// Every lane evaluates its points against the LDS list with the record
+// broadcast from LDS and the pair parameters in registers; in FP32 grid
+// tiles each wave first re-sorts the shell records against its own, smaller
+// sphere.
 #include "er_kernels.h"
 
 #include <float.h>

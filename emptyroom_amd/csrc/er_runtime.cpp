@@ -410,9 +410,9 @@ void choose_grid_tile(const FrameworkDev *fw, const int n[3], const Tuning &tu, 
 }
 
 // Bins of near-cubic shape (per perpendicular width) whose mean occupancy is
-// at most widom_per_bin: the tile kernel takes a bin's insertions 256 at a
-// time, so a mean just below a multiple of 256 keeps the last chunk full
-// while bins above the mean stay rare.  Refines the rounded counts: adds bins
+// at most widom_per_bin: the tile kernel takes a bin's insertions 256 x
+// (points per lane, at most 2) at a time, so a mean just below a multiple of
+// that keeps the last chunk full while bins above the mean stay rare.  Refines the rounded counts: adds bins
 // along the axis with the widest bins until the mean fits, then removes any
 // bin whose removal keeps it fitting.
 void choose_widom_bins(const FrameworkDev *fw, uint64_t n_ins, const Tuning &tu, int nb[3]) {
