@@ -212,26 +212,42 @@ int label_chunk(Context *ctx, GridDev *const *grids, int ng, double temperature,
     // src/pocketblocking.cc:130: E >= 15 T is inaccessible
     HIP_TRY(launch_cc_label(d_tab, ng, blocks, bricks, fblocks, 15.0 * temperature, packed, meta,
                             s));
-    std::vector<unsigned int> hmeta(4 * (size_t)ng);
-    HIP_TRY(hipMemcpyAsync(hmeta.data(), meta, sizeof(unsigned int) * 4 * ng,
-                           hipMemcpyDeviceToHost, s));
+    // One pinned staging buffer: the per-grid meta and the first kLinkPrefix
+    // packed links come back with one synchronisation; a larger link set
+    // takes a second copy.
+    constexpr long long kLinkPrefix = 1 << 18;
+    const long long prefix = std::min<long long>(kLinkPrefix, std::max<long long>(face_sum, 1));
+    const size_t meta_bytes = sizeof(unsigned int) * 4 * (size_t)ng;
+    const size_t stage_bytes = ((meta_bytes + 15) & ~(size_t)15) + sizeof(int) * 3 * prefix;
+    char *stage = (char *)ctx->host_stage(stage_bytes);
+    if (!stage) return ST_NOMEM;
+    unsigned int *hmeta = (unsigned int *)stage;
+    int *hl = (int *)(stage + ((meta_bytes + 15) & ~(size_t)15));
+    HIP_TRY(hipMemcpyAsync(hmeta, meta, meta_bytes, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipMemcpyAsync(hl, packed, sizeof(int) * 3 * prefix, hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     long long nlinks = 0;
     for (int g = 0; g < ng; g++) nlinks += hmeta[4 * g];
-    std::vector<int> hl(3 * (size_t)std::max<long long>(nlinks, 1));
-    if (nlinks > 0) {
-        HIP_TRY(hipMemcpyAsync(hl.data(), packed, sizeof(int) * 3 * nlinks,
+    std::vector<int> hl_big;
+    if (nlinks > prefix) {
+        hl_big.resize(3 * (size_t)nlinks);
+        HIP_TRY(hipMemcpyAsync(hl_big.data(), packed, sizeof(int) * 3 * nlinks,
                                hipMemcpyDeviceToHost, s));
         HIP_TRY(hipStreamSynchronize(s));
+        hl = hl_big.data();
     }
-    // classify every grid (independent; spread over host threads)
+    // classify every grid (independent; a large link set is spread over host
+    // threads, a small one is cheaper than starting them)
     std::vector<std::vector<uint8_t>> pocket(ng);
     std::vector<int> np(ng, 0);
     auto work = [&](int g) {
-        np[g] = classify_pockets((int)hmeta[4 * g + 1], hl.data() + 3 * (size_t)hmeta[4 * g + 2],
+        np[g] = classify_pockets((int)hmeta[4 * g + 1], hl + 3 * (size_t)hmeta[4 * g + 2],
                                  hmeta[4 * g], &pocket[g]);
     };
-    const int nthreads = std::min<int>(ng, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
+    const int nthreads =
+        nlinks < 100000
+            ? 1
+            : std::min<int>(ng, std::max(1u, std::min(16u, std::thread::hardware_concurrency())));
     if (nthreads <= 1) {
         for (int g = 0; g < ng; g++) work(g);
     } else {

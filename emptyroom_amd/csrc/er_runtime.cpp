@@ -165,6 +165,25 @@ Context::~Context() {
     pool.free(d_partials);
     pool.free(d_sums);
     pool.free(d_scratch);
+    if (h_stage) (void)hipHostFree(h_stage);
+}
+
+void *Context::host_stage(size_t bytes) {
+    if (bytes <= h_stage_bytes) return h_stage;
+    if (h_stage) {
+        (void)hipHostFree(h_stage);
+        h_stage = nullptr;
+        h_stage_bytes = 0;
+    }
+    const size_t b = std::max<size_t>(bytes, 1 << 20);
+    if (hipHostMalloc(&h_stage, b) != hipSuccess) {
+        (void)hipGetLastError();
+        h_stage = nullptr;
+        set_error("pinned host staging allocation failed");
+        return nullptr;
+    }
+    h_stage_bytes = b;
+    return h_stage;
 }
 
 int context_create(int device, size_t hbm_cap, Context **out) {

@@ -98,7 +98,12 @@ class Context {
     double *d_sums = nullptr;
     double4 *d_scratch = nullptr;  // region sub-lists, scratch_wgs * kRegionRecords records
     int scratch_wgs = 0;
+    void *h_stage = nullptr;       // pinned host staging of pocket labelling results
+    size_t h_stage_bytes = 0;
     ~Context();
+    // Pinned host buffer of at least `bytes` (kept, grown on demand); nullptr
+    // with the error set if the allocation fails.
+    void *host_stage(size_t bytes);
 };
 
 int context_create(int device, size_t hbm_cap, Context **out);
