@@ -451,13 +451,16 @@ def main():
     achieved = FLOP_PER_PAIR * incut / kern_s / 1e12
     log(f"in-cutoff pairs {incut:.4g}; kernel {achieved:.2f} TFLOP/s")
 
-    cpu = None
+    cpu = cpu1 = None
     grid_err = kh_err = e_err = None
     if not args.no_cpu:
         threads = min(16, os.cpu_count() or 1)
         cpu, grid_err, kh_err, e_err = cpu_baseline(fws, uploaded, ctx, batch, args.cpu_seconds,
                                                     threads)
         log(f"cpu baseline {cpu['value']:.3g} pair-evals/s on {threads} threads")
+        # the same kind of sample on one core (a third of the time budget)
+        cpu1, _, _, _ = cpu_baseline(fws, uploaded, ctx, batch, args.cpu_seconds / 3.0, 1)
+        log(f"cpu baseline {cpu1['value']:.3g} pair-evals/s on 1 thread")
 
     value = (grid_pe + widom_pe) / t_batched
     out = {
@@ -495,6 +498,7 @@ def main():
                      "flop_per_pair": FLOP_PER_PAIR},
         "tuning": tuning,
         "cpu_baseline": cpu,
+        "cpu_baseline_1core": cpu1,
     }
     print(json.dumps(out))
     batch.close()

@@ -1,0 +1,72 @@
+"""Parity at the batching mix's full sizes, through properties that do not
+need the oracle to visit every point.
+
+Framework cfg3_020 of the 256-framework mix (110 atoms, triclinic, 0.1 A
+spacing, 1.14e7 grid points, images 3 x 3 x 7; about a quarter of its points
+are accessible -- the denser frameworks of the mix are blocked almost
+everywhere, which would leave nothing to compare):
+- the FP32 grid is within 1e-3 K of the FP64 grid at every point with
+  E < 1e4 K (relative 1e-6 beyond), over all 1.14e7 points;
+- the FP64 grid matches the oracle to 1e-9 relative at 3000 random points;
+- 1e7 Widom insertions: the energies of the first 20000 match the oracle to
+  1e-3 K, every energy is finite or a clamped overlap, and the sums are the
+  ones the returned energies give (both sums recomputed on the host in FP64
+  agree to 1e-12 of the sum of magnitudes).
+"""
+import numpy as np
+import pytest
+
+from emptyroom_amd import api, synth
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def fw_020(ctx):
+    f = synth.config3(256)[20]
+    assert f.name == "cfg3_020" and f.spacing == 0.1
+    a = f.particles()
+    fw = api.Framework(ctx, a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2)
+    yield f, a, fw
+    fw.close()
+
+
+def test_full_grid_fp32_vs_fp64_and_oracle(ctx, fw_020):
+    f, a, fw = fw_020
+    n = f.grid_n()
+    print(f"full grid {n} ({np.prod(n):.3g} points)", flush=True)
+    ctx.tuning = api.default_tuning()
+    g64 = api.compute_grid(ctx, fw, n, api.ER_FP64)
+    g32 = api.compute_grid(ctx, fw, n, api.ER_FP32).astype(np.float64)
+    lo = g64 < 1e4
+    assert lo.sum() > 1e6  # a porous framework: many accessible points
+    err = float(np.max(np.abs(g32[lo] - g64[lo])))
+    assert err <= 1e-3, err
+    hi = ~lo & (g64 < 3.0e38)
+    if hi.any():
+        assert float(np.max(np.abs(g32[hi] - g64[hi]) / g64[hi])) < 1e-6
+    rng = np.random.default_rng(3)
+    idx = np.stack([rng.integers(0, n[d], 3000) for d in range(3)], 1)
+    fpts = idx / (np.array(n) - 1.0)
+    ref = oracle.lj_points(f.t, a, f.rep, f.rc2, 1e-24, fpts)
+    got = g64[idx[:, 0], idx[:, 1], idx[:, 2]]
+    rel = np.abs(got - ref) / np.maximum(1.0, np.abs(ref))
+    assert rel.max() < 1e-9, rel.max()
+
+
+def test_widom_1e7_sample_and_sums(ctx, fw_020):
+    f, a, fw = fw_020
+    nins, T, seed = 10_000_000, 298.0, 77
+    print(f"widom {nins} insertions", flush=True)
+    ctx.tuning = api.default_tuning()
+    sb, seb, n, e = api.widom(ctx, fw, nins, T, seed, energies=True)
+    assert n == nins
+    assert np.all(np.isfinite(e) | (e > 0))
+    _, _, re = oracle.widom(f.t, a, f.rep, f.rc2, T, seed, 0, 20000, energies=True)
+    m = re < 1e4
+    assert np.max(np.abs(e[:20000][m] - re[m])) <= 1e-3
+    b = np.exp(-e / T)
+    assert abs(b.sum() - sb) <= 1e-12 * sb
+    eb = np.where(b == 0.0, 0.0, e * b)
+    assert abs(eb.sum() - seb) <= 1e-12 * np.abs(eb).sum()
