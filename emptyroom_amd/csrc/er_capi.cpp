@@ -72,6 +72,8 @@ int er_context_destroy(er_context *ctx) {
     return ST_OK;
 }
 
+}  // extern "C"
+
 namespace {
 void tuning_to(const er::Tuning &t, er_tuning *out) {
     out->points_per_lane = t.ppl;
@@ -86,6 +88,8 @@ void tuning_to(const er::Tuning &t, er_tuning *out) {
 }
 }  // namespace
 
+extern "C" {
+
 int er_context_get_tuning(const er_context *ctx, er_tuning *out) {
     if (!ctx || !out) return arg_error("er_context_get_tuning: null argument");
     tuning_to(C(ctx)->tuning, out);
@@ -97,6 +101,8 @@ int er_tuning_defaults(er_tuning *out) {
     tuning_to(er::Tuning(), out);
     return ST_OK;
 }
+
+}  // extern "C"
 
 namespace {
 er::Tuning tuning_from(const er_tuning *in) {
@@ -113,6 +119,8 @@ er::Tuning tuning_from(const er_tuning *in) {
     return t;
 }
 }  // namespace
+
+extern "C" {
 
 int er_grid_tile_for(const er_tuning *tuning, const double t_matrix[9], const int n[3],
                      int tile_out[3], int *size_class) {
