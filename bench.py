@@ -31,6 +31,14 @@ sys.path.insert(0, ROOT)
 
 PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector/matrix peak (packed FP32)
 DEFAULT_TUNING_FILE = os.path.join(ROOT, "emptyroom_amd", "tuning", "mi355x_config3.json")
+WORKLOAD_TEXT = {
+    "config3": "config3: {n} frameworks, grids FP32 at 0.1/0.2/0.5 A + 1e6 Widom insertions "
+               "each, one batched launch",
+    "config2": "config2: one 3000-atom triclinic framework, FP32 grid at 0.1 A + 1e7 Widom "
+               "insertions, one batched launch",
+    "config1": "config1: one 400-atom cubic framework, grid at 0.5 A (FP32 here) + 1e5 Widom "
+               "insertions, one batched launch",
+}
 FLOP_PER_PAIR = 16
 
 
@@ -476,8 +484,8 @@ def main():
         "vs_baseline": None,
         "dtype": "fp32 pair LJ beyond 2 sigma, fp64 r^2 / close pairs / accumulation",
         "data": "synthetic (seeded SplitMix64 frameworks)",
-        "config": {"workload": f"{args.workload}: {len(fws)} frameworks, grids FP32 at "
-                               "0.1/0.2/0.5 A + 1e6 Widom insertions each, one batched launch",
+        "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload).format(
+                       n=len(fws)),
                    "frameworks": len(fws), "grid_points": points, "insertions": n_ins,
                    "parallelism": "single GPU"},
         "inserts_per_s": n_ins / t_batched,
