@@ -190,8 +190,10 @@ __global__ __launch_bounds__(kWG) void k_cc_local(const CCGrid *__restrict__ gri
 }
 
 // Unions across brick faces: every accessible voxel on a brick's -x, -y or
-// -z face with its accessible neighbour in the adjacent brick.
-__global__ __launch_bounds__(kWG) void k_cc_merge(const CCGrid *__restrict__ grids, int ng) {
+// -z face with its accessible neighbour in the adjacent brick.  One wave per
+// brick walks the 169 face voxels: the -z face (64), then the rest of the -y
+// face (8 x 7), then the rest of the -x face (7 x 7).
+__global__ __launch_bounds__(64) void k_cc_merge(const CCGrid *__restrict__ grids, int ng) {
     const CCGrid &G = grids[cc_brick_of(grids, ng, blockIdx.x)];
     const int nx = G.n[0], ny = G.n[1], nz = G.n[2];
     const int nbz = (nz + 7) >> 3, nby = (ny + 7) >> 3;
@@ -199,11 +201,25 @@ __global__ __launch_bounds__(kWG) void k_cc_merge(const CCGrid *__restrict__ gri
     const int bk = (int)(b % nbz), bj = (int)((b / nbz) % nby), bi = (int)(b / ((long long)nbz * nby));
     const long long syz = (long long)ny * nz;
     int *parent = G.parent;
+    const int t = threadIdx.x;
 #pragma unroll
-    for (int h = 0; h < kBrick / kWG; ++h) {
-        const int l = threadIdx.x + h * kWG;
-        const int ii = l >> 6, jj = (l >> 3) & 7, kk = l & 7;
-        if (ii != 0 && jj != 0 && kk != 0) continue;
+    for (int face = 0; face < 3; ++face) {
+        int ii, jj, kk;
+        if (face == 0) {
+            ii = t >> 3;
+            jj = t & 7;
+            kk = 0;
+        } else if (face == 1) {
+            if (t >= 56) continue;
+            ii = t / 7;
+            jj = 0;
+            kk = 1 + t % 7;
+        } else {
+            if (t >= 49) continue;
+            ii = 0;
+            jj = 1 + t / 7;
+            kk = 1 + t % 7;
+        }
         const int i = bi * 8 + ii, j = bj * 8 + jj, k = bk * 8 + kk;
         if (i >= nx || j >= ny || k >= nz) continue;
         const long long v = (long long)k + (long long)j * nz + (long long)i * syz;
@@ -532,7 +548,7 @@ hipError_t launch_cc_label(const CCGrid *grids, int ng, long long total_blocks,
         return hipErrorInvalidValue;
     const dim3 nb((unsigned)total_blocks);
     hipLaunchKernelGGL(k_cc_local, dim3((unsigned)total_bricks), dim3(kWG), 0, s, grids, ng, thr);
-    hipLaunchKernelGGL(k_cc_merge, dim3((unsigned)total_bricks), dim3(kWG), 0, s, grids, ng);
+    hipLaunchKernelGGL(k_cc_merge, dim3((unsigned)total_bricks), dim3(64), 0, s, grids, ng);
     hipLaunchKernelGGL(k_cc_flatten, nb, dim3(kWG), 0, s, grids, ng);
     hipLaunchKernelGGL(k_cc_word_scan, dim3(ng), dim3(kWG), 0, s, grids);
     hipLaunchKernelGGL(k_cc_zero, dim3(1), dim3(kWG), 0, s, meta, 4 * ng);
