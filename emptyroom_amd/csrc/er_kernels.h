@@ -84,6 +84,17 @@ hipError_t launch_widom_partials(const double *e, long long n, double temperatur
 hipError_t launch_widom_final_n(const double2 *partials, int nblk, double *out, hipStream_t s);
 // out[0] = sum_b, out[1] = sum_Eb: fixed-order sum of the kReduceBlocks partials.
 hipError_t launch_widom_final(const double2 *partials, double *out, hipStream_t s);
+// One Widom run's reduction: energies e[0, n), kReduceBlocks partials, two sums.
+struct WidomReduce {
+    const double *e;
+    long long n;
+    double temperature;
+    double2 *partials;
+    double *out;
+};
+// The partials and final sums of nruns runs (device table) in two launches,
+// each run summed exactly as launch_widom_partials + launch_widom_final do.
+hipError_t launch_widom_reduce_many(const WidomReduce *runs, int nruns, hipStream_t s);
 // Sum of n uint32 counts into *out (uint64, must be zero).
 hipError_t launch_sum_u32(const uint32_t *v, long long n, unsigned long long *out, hipStream_t s);
 // Sum of n FP64-stored integer counts into *out (uint64, must be zero).

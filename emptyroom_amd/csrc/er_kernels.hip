@@ -1081,10 +1081,13 @@ __device__ __forceinline__ double wave_sum(double v) {
 
 // Block g sums the Boltzmann factors b = exp(-E/T) (src/doHenry.h:107) and
 // E*b over the index range [g*n/G, (g+1)*n/G) in a fixed order.
-__global__ __launch_bounds__(kWG) void k_widom_partials(const double *e, long long n,
-                                                        double temperature, double2 *partials) {
-    __shared__ double s_b[kWG / 64], s_eb[kWG / 64];
-    const long long g = blockIdx.x, G = gridDim.x;
+// Block g of G sums exp(-E/T) and E exp(-E/T) over the contiguous insertion
+// range [g n / G, (g+1) n / G) in a fixed order (thread stride, then a fixed
+// shuffle tree and the four wave sums in order).
+__device__ __forceinline__ void widom_partial_block(const double *e, long long n,
+                                                   double temperature, long long g, long long G,
+                                                   double2 *partials, double *s_b,
+                                                   double *s_eb) {
     const long long lo = g * n / G, hi = (g + 1) * n / G;
     double sb = 0.0, seb = 0.0;
     for (long long i = lo + threadIdx.x; i < hi; i += kWG) {
@@ -1108,9 +1111,8 @@ __global__ __launch_bounds__(kWG) void k_widom_partials(const double *e, long lo
 // One block of kWG threads: thread t sums partials t, t + kWG, ... in order,
 // then a fixed shuffle tree per wave and the wave sums in order -- the same
 // summation order on every run.
-__global__ __launch_bounds__(kWG) void k_widom_final(const double2 *partials, int nblk,
-                                                     double *out) {
-    __shared__ double s_b[kWG / 64], s_eb[kWG / 64];
+__device__ __forceinline__ void widom_final_block(const double2 *partials, int nblk, double *out,
+                                                  double *s_b, double *s_eb) {
     double sb = 0.0, seb = 0.0;
     for (int g = threadIdx.x; g < nblk; g += kWG) {
         sb += partials[g].x;
@@ -1127,6 +1129,32 @@ __global__ __launch_bounds__(kWG) void k_widom_final(const double2 *partials, in
         out[0] = (s_b[0] + s_b[1]) + (s_b[2] + s_b[3]);
         out[1] = (s_eb[0] + s_eb[1]) + (s_eb[2] + s_eb[3]);
     }
+}
+
+__global__ __launch_bounds__(kWG) void k_widom_partials(const double *e, long long n,
+                                                        double temperature, double2 *partials) {
+    __shared__ double s_b[kWG / 64], s_eb[kWG / 64];
+    widom_partial_block(e, n, temperature, blockIdx.x, gridDim.x, partials, s_b, s_eb);
+}
+
+__global__ __launch_bounds__(kWG) void k_widom_final(const double2 *partials, int nblk,
+                                                     double *out) {
+    __shared__ double s_b[kWG / 64], s_eb[kWG / 64];
+    widom_final_block(partials, nblk, out, s_b, s_eb);
+}
+
+// Every Widom run of a batch in one launch: block b serves run b / kReduceBlocks.
+__global__ __launch_bounds__(kWG) void k_widom_partials_many(const WidomReduce *runs) {
+    __shared__ double s_b[kWG / 64], s_eb[kWG / 64];
+    const WidomReduce &R = runs[blockIdx.x / kReduceBlocks];
+    widom_partial_block(R.e, R.n, R.temperature, blockIdx.x % kReduceBlocks, kReduceBlocks,
+                        R.partials, s_b, s_eb);
+}
+
+__global__ __launch_bounds__(kWG) void k_widom_final_many(const WidomReduce *runs) {
+    __shared__ double s_b[kWG / 64], s_eb[kWG / 64];
+    const WidomReduce &R = runs[blockIdx.x];
+    widom_final_block(R.partials, kReduceBlocks, R.out, s_b, s_eb);
 }
 
 __global__ __launch_bounds__(kWG) void k_sum_u32(const uint32_t *v, long long n,
@@ -1259,6 +1287,14 @@ hipError_t launch_widom_partials(const double *e, long long n, double temperatur
                                  double2 *partials, hipStream_t s) {
     hipLaunchKernelGGL(k_widom_partials, dim3(kReduceBlocks), dim3(kWG), 0, s, e, n, temperature,
                        partials);
+    return hipGetLastError();
+}
+
+hipError_t launch_widom_reduce_many(const WidomReduce *runs, int nruns, hipStream_t s) {
+    if (nruns <= 0) return hipSuccess;
+    hipLaunchKernelGGL(k_widom_partials_many, dim3((unsigned)nruns * kReduceBlocks), dim3(kWG), 0,
+                       s, runs);
+    hipLaunchKernelGGL(k_widom_final_many, dim3((unsigned)nruns), dim3(kWG), 0, s, runs);
     return hipGetLastError();
 }
 

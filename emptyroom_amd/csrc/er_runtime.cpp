@@ -961,12 +961,8 @@ int Batch::record(hipStream_t s) {
         if (ev_on) HIP_TRY(hipEventRecord(ev[3], s));
         timed_launches |= 2;
     }
-    for (size_t t = 0; t < tasks.size(); t++)
-        if (tasks[t].n_ins > 0) {
-            HIP_TRY(launch_widom_partials(wbufs[t].e, (long long)tasks[t].n_ins,
-                                          tasks[t].temperature, d_partials + t * kReduceBlocks, s));
-            HIP_TRY(launch_widom_final(d_partials + t * kReduceBlocks, d_sums + 2 * t, s));
-        }
+    // every Widom run's sums in two launches (same per-run order as one run alone)
+    HIP_TRY(launch_widom_reduce_many(d_reduce, (int)reduce.size(), s));
     return ST_OK;
 }
 
@@ -1062,8 +1058,14 @@ int batch_create(Context *ctx, const Task *tasks, int n, bool use_graph, Batch *
         (st = pool_alloc(ctx, (size_t)std::max(n, 1) * 2, &b->d_sums, &b->owned)) ||
         (st = pool_alloc(ctx, 2, &b->d_counter, &b->owned)) ||
         (st = pool_alloc(ctx, std::max<size_t>(b->bin_jobs.size(), 1), &b->d_bin_jobs,
-                         &b->owned)))
+                         &b->owned)) ||
+        (st = pool_alloc(ctx, (size_t)std::max(n, 1), &b->d_reduce, &b->owned)))
         return fail(st);
+    for (int t = 0; t < n; t++)
+        if (b->tasks[t].n_ins > 0)
+            b->reduce.push_back({b->wbufs[t].e, (long long)b->tasks[t].n_ins,
+                                 b->tasks[t].temperature, b->d_partials + (size_t)t * kReduceBlocks,
+                                 b->d_sums + 2 * (size_t)t});
     if (hipHostMalloc((void **)&b->h_sums, sizeof(double) * 2 * std::max(n, 1)) != hipSuccess) {
         set_error("batch_create: pinned host allocation failed");
         return fail(ST_NOMEM);
@@ -1073,6 +1075,9 @@ int batch_create(Context *ctx, const Task *tasks, int n, bool use_graph, Batch *
                         hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
         (!b->bin_jobs.empty() &&
          hipMemcpyAsync(b->d_bin_jobs, b->bin_jobs.data(), sizeof(BinJob) * b->bin_jobs.size(),
+                        hipMemcpyHostToDevice, ctx->stream) != hipSuccess) ||
+        (!b->reduce.empty() &&
+         hipMemcpyAsync(b->d_reduce, b->reduce.data(), sizeof(WidomReduce) * b->reduce.size(),
                         hipMemcpyHostToDevice, ctx->stream) != hipSuccess)) {
         set_error("batch_create: job upload failed");
         return fail(ST_HIP);

@@ -236,6 +236,8 @@ class Batch {
     int math = MATH_MIXED;
     double2 *d_partials = nullptr;     // kReduceBlocks per task
     double *d_sums = nullptr;          // 2 per task
+    std::vector<WidomReduce> reduce;   // reduction of every task with insertions
+    WidomReduce *d_reduce = nullptr;
     unsigned long long *d_counter = nullptr;
     double *h_sums = nullptr;          // pinned
     hipGraph_t graph = nullptr;
