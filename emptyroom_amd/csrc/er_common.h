@@ -123,7 +123,7 @@ struct BinJob {
 constexpr int kWG = 256;      // threads per workgroup (4 wave64)
 constexpr int kMaxSeg = 64;   // list segments per fill pass
 constexpr int kMaxTypes = 64; // pair types a region sub-list can index
-constexpr int kLdsBins = 8192; // Widom bins a binning block histograms in LDS
+constexpr int kLdsBinsMax = 32768; // Widom bins a binning block histograms in LDS (128 KiB)
 constexpr int kBinChunk = 16384;  // insertions per binning block
 constexpr int kRegionRecords = 8192;  // records of one workgroup's region sub-list (256 KiB)
 constexpr long long kMinGridItems = 2048;  // regions per grid before regions shrink

@@ -186,8 +186,9 @@ __device__ __forceinline__ uint32_t widom_bin(const double f[3], const int nb[3]
 // range of the bin's slots and hands them out through LDS atomics.  The order
 // inside a bin is therefore arbitrary; no result depends on it (every
 // insertion of a bin sees the same candidate list, and sums run over
-// insertion indices).  Jobs with more than kLdsBins bins use one global
-// atomic per insertion.
+// insertion indices).  The histogram lives in dynamic LDS sized per launch
+// for the largest job's bins (lds_bins, at most kLdsBinsMax); jobs with more
+// bins than that use one global atomic per insertion.
 __device__ __forceinline__ int bin_job_of(const BinJob *__restrict__ jobs, int njobs,
                                           long long blk) {
     int lo = 0, hi = njobs - 1;
@@ -206,14 +207,14 @@ __global__ __launch_bounds__(kWG) void k_bin_zero(const BinJob *__restrict__ job
 }
 
 __global__ __launch_bounds__(kWG) void k_bin_count(const BinJob *__restrict__ jobs, int njobs,
-                                                   int chunk) {
-    __shared__ uint32_t s_hist[kLdsBins];
+                                                   int chunk, int lds_bins) {
+    extern __shared__ uint32_t s_hist[];
     const int j = bin_job_of(jobs, njobs, blockIdx.x);
     const BinJob &J = jobs[j];
     const uint64_t i0 = (uint64_t)(blockIdx.x - J.block0) * (uint64_t)chunk;
     const uint64_t i1 = i0 + (uint64_t)chunk < J.n ? i0 + (uint64_t)chunk : J.n;
     const int nbt = J.nb[0] * J.nb[1] * J.nb[2];
-    if (nbt > kLdsBins) {
+    if (nbt > lds_bins) {
         for (uint64_t i = i0 + threadIdx.x; i < i1; i += kWG) {
             double f[3];
             widom_point(J.seed, J.stream, i, f);
@@ -240,14 +241,14 @@ __global__ __launch_bounds__(kWG) void k_bin_scan(const BinJob *__restrict__ job
 }
 
 __global__ __launch_bounds__(kWG) void k_bin_scatter(const BinJob *__restrict__ jobs, int njobs,
-                                                     int chunk) {
-    __shared__ uint32_t s_hist[kLdsBins];
+                                                     int chunk, int lds_bins) {
+    extern __shared__ uint32_t s_hist[];
     const int j = bin_job_of(jobs, njobs, blockIdx.x);
     const BinJob &J = jobs[j];
     const uint64_t i0 = (uint64_t)(blockIdx.x - J.block0) * (uint64_t)chunk;
     const uint64_t i1 = i0 + (uint64_t)chunk < J.n ? i0 + (uint64_t)chunk : J.n;
     const int nbt = J.nb[0] * J.nb[1] * J.nb[2];
-    if (nbt > kLdsBins) {
+    if (nbt > lds_bins) {
         for (uint64_t i = i0 + threadIdx.x; i < i1; i += kWG) {
             double f[3];
             widom_point(J.seed, J.stream, i, f);
@@ -1240,12 +1241,29 @@ hipError_t launch_bin_all(const BinJob *jobs, int njobs, long long total_blocks,
                           long long max_bins, hipStream_t s) {
     if (njobs <= 0 || total_blocks <= 0) return hipSuccess;
     if (chunk < kWG || total_blocks > (1LL << 31) - 1 || max_bins <= 0) return hipErrorInvalidValue;
+    // histogram in LDS for every job when the largest fits, sized for it
+    int lds_bins = max_bins <= kLdsBinsMax ? (int)max_bins : 0;
+    if (lds_bins > 16384) {
+        // more than 64 KiB of dynamic LDS must be allowed per kernel
+        static const bool big_lds_ok =
+            hipFuncSetAttribute((const void *)k_bin_count,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                kLdsBinsMax * (int)sizeof(uint32_t)) == hipSuccess &&
+            hipFuncSetAttribute((const void *)k_bin_scatter,
+                                hipFuncAttributeMaxDynamicSharedMemorySize,
+                                kLdsBinsMax * (int)sizeof(uint32_t)) == hipSuccess;
+        if (!big_lds_ok) {
+            (void)hipGetLastError();
+            lds_bins = 0;
+        }
+    }
+    const size_t lds_bytes = sizeof(uint32_t) * (size_t)(lds_bins > 0 ? lds_bins : 1);
     hipLaunchKernelGGL(k_bin_zero, dim3(njobs), dim3(kWG), 0, s, jobs);
-    hipLaunchKernelGGL(k_bin_count, dim3((unsigned)total_blocks), dim3(kWG), 0, s, jobs, njobs,
-                       chunk);
+    hipLaunchKernelGGL(k_bin_count, dim3((unsigned)total_blocks), dim3(kWG), lds_bytes, s, jobs,
+                       njobs, chunk, lds_bins);
     hipLaunchKernelGGL(k_bin_scan, dim3(njobs), dim3(kWG), 0, s, jobs);
-    hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)total_blocks), dim3(kWG), 0, s, jobs, njobs,
-                       chunk);
+    hipLaunchKernelGGL(k_bin_scatter, dim3((unsigned)total_blocks), dim3(kWG), lds_bytes, s, jobs,
+                       njobs, chunk, lds_bins);
     return hipGetLastError();
 }
 
