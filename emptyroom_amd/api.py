@@ -138,7 +138,7 @@ EXPORTS = [
     "er_host_run_write_settings", "er_host_run_release", "er_write_grid_file",
     "er_replication_factors", "er_write_uc_file", "er_accessibility_grid",
     "er_grid_file_info", "er_read_grid_file", "er_write_grid_binary", "er_grid_load",
-    "er_tuning_defaults", "er_grid_tile_for",
+    "er_tuning_defaults", "er_grid_tile_for", "er_grid_energies",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -218,6 +218,8 @@ _SIGS = {
     "er_accessibility_grid": (ctypes.c_int, [_P, _P, ctypes.c_int, _P,
                                              ctypes.POINTER(ctypes.c_int), _P]),
     "er_tuning_defaults": (ctypes.c_int, [ctypes.POINTER(Tuning)]),
+    "er_grid_energies": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_uint64,
+                                        ctypes.POINTER(ctypes.c_double)]),
     "er_grid_tile_for": (ctypes.c_int, [ctypes.POINTER(Tuning), ctypes.POINTER(ctypes.c_double),
                                         ctypes.POINTER(ctypes.c_int), ctypes.POINTER(ctypes.c_int),
                                         ctypes.POINTER(ctypes.c_int)]),
@@ -526,6 +528,15 @@ def pockets_label_batch(ctx: Context, grids: Sequence[Grid], temperature: float,
     _check(lib().er_pockets_label_batch(ctx.h, arr, len(grids), float(temperature), int(block),
                                         out))
     return [out[i] for i in range(len(grids))]
+
+
+def grid_energies(ctx: Context, grid: Grid, frac: np.ndarray) -> np.ndarray:
+    """Guest-framework energies (K) at fractional positions frac[n, 3] (any
+    range; wrapped into [0, 1)) by trilinear interpolation of the grid."""
+    f = np.ascontiguousarray(np.asarray(frac, dtype=np.float64).reshape(-1, 3))
+    out = np.empty(len(f), dtype=np.float64)
+    _check(lib().er_grid_energies(ctx.h, grid.h, _dptr(f), len(f), _dptr(out)))
+    return out
 
 
 def widom_grid(ctx: Context, grid: Grid, n: int, temperature: float, seed: int,

@@ -369,6 +369,15 @@ int er_pockets_label_batch(er_context *ctx, er_grid *const *grids, int n_grids,
     });
 }
 
+int er_grid_energies(er_context *ctx, const er_grid *grid, const double *frac_xyz,
+                     uint64_t n_positions, double *energies_out) {
+    if (!ctx || !grid) return arg_error("er_grid_energies: null argument");
+    return guarded([&]() -> int {
+        return er::grid_energies(C(ctx), const_cast<er::GridDev *>(G(grid)), frac_xyz,
+                                 n_positions, energies_out);
+    });
+}
+
 int er_widom_grid(er_context *ctx, er_grid *grid, uint64_t n_insertions, double temperature,
                   uint64_t seed, er_widom_result *out, double *energies_out) {
     if (!ctx || !grid || !out) return arg_error("er_widom_grid: null argument");

@@ -48,6 +48,10 @@ hipError_t launch_cc_block(const CCGrid *grids, int ng, long long total_blocks, 
 // Grid-interpolated Widom with fused Boltzmann sums: partials (kTriBlocks
 // entries) receive fixed-order per-block (sum b, sum E b); e_out nullable.
 constexpr int kTriBlocks = 4096;
+// Energies (K) at npos fractional positions f[3 npos] (wrapped into [0, 1)),
+// interpolated on an FP32 (precision 0) or FP64 (precision 1) grid.
+hipError_t launch_grid_energies(const void *g, int precision, const int n[3], const double *f,
+                                long long npos, double *out, hipStream_t s);
 hipError_t launch_widom_grid(const float *g, const int n[3], uint64_t seed, uint32_t stream,
                              uint64_t nins, double temperature, double *e_out, double2 *partials,
                              hipStream_t s);

@@ -17,6 +17,8 @@
 // pass overwrites pocket voxels with 1e13.
 #include "er_pockets.h"
 
+#include <algorithm>
+
 namespace er {
 
 namespace {
@@ -458,7 +460,8 @@ __global__ __launch_bounds__(kWG) void k_cc_block(const CCGrid *__restrict__ gri
 
 // Trilinear energy of src/gcmc.cc:145-177 after the wrap of src/gcmc.cc:74-77,
 // with the lower index clamped to N-2 (f = 1 reads node N-1).
-__device__ __forceinline__ double trilinear(const float *__restrict__ g, int nx, int ny, int nz,
+template <class T>
+__device__ __forceinline__ double trilinear(const T *__restrict__ g, int nx, int ny, int nz,
                                             double xf, double yf, double zf) {
     // Separate multiplies and adds, as the reference's host code computes
     // them: next to blocked (1e13 K) nodes a fused multiply-add changes the
@@ -484,6 +487,20 @@ __device__ __forceinline__ double trilinear(const float *__restrict__ g, int nx,
     const double c0 = c00 * (1.0 - yd) + c10 * yd;
     const double c1 = c01 * (1.0 - yd) + c11 * yd;
     return c0 * (1 - zd) + c1 * zd;
+}
+
+// Guest-framework energy at caller-given fractional positions: the wrap of
+// src/gcmc.cc:74-77 (x - floor(x)) and the interpolation of
+// src/gcmc.cc:145-177 on an FP32 or FP64 grid, one position per thread.
+template <class T>
+__global__ __launch_bounds__(kWG) void k_grid_energies(const T *__restrict__ g, int nx, int ny,
+                                                       int nz, const double *__restrict__ f,
+                                                       long long n, double *__restrict__ out) {
+    for (long long i = (long long)blockIdx.x * kWG + threadIdx.x; i < n;
+         i += (long long)gridDim.x * kWG) {
+        const double x = f[3 * i], y = f[3 * i + 1], z = f[3 * i + 2];
+        out[i] = trilinear(g, nx, ny, nz, x - floor(x), y - floor(y), z - floor(z));
+    }
 }
 
 // Grid-interpolated insertions with the Boltzmann sums fused in: block b of
@@ -526,6 +543,19 @@ __global__ __launch_bounds__(kWG) void k_widom_grid(const float *__restrict__ g,
 }
 
 }  // namespace
+
+hipError_t launch_grid_energies(const void *g, int precision, const int n[3], const double *f,
+                                long long npos, double *out, hipStream_t s) {
+    if (npos <= 0) return hipSuccess;
+    const unsigned blocks = (unsigned)std::min<long long>((npos + kWG - 1) / kWG, 65536);
+    if (precision == 1)
+        hipLaunchKernelGGL(k_grid_energies<double>, dim3(blocks), dim3(kWG), 0, s,
+                           (const double *)g, n[0], n[1], n[2], f, npos, out);
+    else
+        hipLaunchKernelGGL(k_grid_energies<float>, dim3(blocks), dim3(kWG), 0, s,
+                           (const float *)g, n[0], n[1], n[2], f, npos, out);
+    return hipGetLastError();
+}
 
 long long cc_blocks(long long nv) { return (nv + kCCVox - 1) / kCCVox; }
 

@@ -390,4 +390,27 @@ int widom_grid_run(Context *ctx, GridDev *g, uint64_t n_ins, double temperature,
     return ST_OK;
 }
 
+int grid_energies(Context *ctx, GridDev *g, const double *frac_host, uint64_t n,
+                  double *out_host) {
+    if (!ctx || !g || (n && (!frac_host || !out_host))) {
+        set_error("grid_energies: null argument");
+        return ST_ARG;
+    }
+    if (n == 0) return ST_OK;
+    if (n >= (1ULL << 40)) {
+        set_error("grid_energies: position count must be below 2^40");
+        return ST_ARG;
+    }
+    HIP_TRY(hipSetDevice(ctx->device));
+    Scratch sc{ctx, {}};
+    double *f = sc.get<double>(3 * n), *e = sc.get<double>(n);
+    if (!f || !e) return ST_NOMEM;
+    hipStream_t s = ctx->stream;
+    HIP_TRY(hipMemcpyAsync(f, frac_host, sizeof(double) * 3 * n, hipMemcpyHostToDevice, s));
+    HIP_TRY(launch_grid_energies(g->d_data, g->precision, g->n, f, (long long)n, e, s));
+    HIP_TRY(hipMemcpyAsync(out_host, e, sizeof(double) * n, hipMemcpyDeviceToHost, s));
+    HIP_TRY(hipStreamSynchronize(s));
+    return ST_OK;
+}
+
 }  // namespace er

@@ -206,6 +206,9 @@ int pockets_label(Context *ctx, GridDev *g, double temperature, bool block, int 
                   int32_t *seg_host);
 int widom_grid_run(Context *ctx, GridDev *g, uint64_t n_ins, double temperature, uint64_t seed,
                    WidomResult *out, double *energies_host);
+// Energies at n host positions (fractional, any range) into host out.
+int grid_energies(Context *ctx, GridDev *g, const double *frac_host, uint64_t n,
+                  double *out_host);
 // Segment classification from face links (3 ints each: from, to, axis 1..3,
 // from < 0 = no link); returns the number of pockets, -1 on bad links.
 int classify_pockets(int nseg, const int *links, long long nlinks, std::vector<uint8_t> *pocket);

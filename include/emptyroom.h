@@ -218,6 +218,13 @@ int er_pockets_label_batch(er_context *ctx, er_grid *const *grids, int n_grids,
                            double temperature, int block_in_place, int *n_pockets);
 int er_widom_grid(er_context *ctx, er_grid *grid, uint64_t n_insertions, double temperature,
                   uint64_t seed, er_widom_result *out, double *energies_out);
+/* Guest-framework energies (K) at n_positions fractional positions
+ * frac_xyz[3 n] of any range, by the wrap of src/gcmc.cc:74-77 and the
+ * trilinear interpolation of GuestFrameworkEnergy (src/gcmc.cc:145-177) on a
+ * resident FP32 or FP64 grid: the energy calls of GuestFrameworkTotalEnergy
+ * (src/gcmc.cc:182-198) for many guests or chains in one call. */
+int er_grid_energies(er_context *ctx, const er_grid *grid, const double *frac_xyz,
+                     uint64_t n_positions, double *energies_out);
 
 /* ---- serving ------------------------------------------------------------------
  * No reference counterpart.  A server owns a scheduler thread on one context:

@@ -200,6 +200,20 @@ def trilinear(grid, xf, yf, zf):
     return lib().orc_trilinear(_d(g), nx, ny, nz, xf, yf, zf)
 
 
+def trilinear_points(grid, frac):
+    """Interpolated energies at fractional positions frac[n, 3] after the
+    wrap x - floor(x) (src/gcmc.cc:74-77, 145-177), one point at a time."""
+    g = _c(grid)
+    nx, ny, nz = g.shape
+    f = _c(frac).reshape(-1, 3)
+    L = lib()
+    out = np.empty(len(f))
+    for i, (x, y, z) in enumerate(f):
+        out[i] = L.orc_trilinear(_d(g), nx, ny, nz, L.orc_wrap(x, 1.0), L.orc_wrap(y, 1.0),
+                                 L.orc_wrap(z, 1.0))
+    return out
+
+
 def widom_grid(grid, temperature, seed, i0, i1, energies=False, nthreads=0):
     g = _c(grid)
     nx, ny, nz = g.shape
