@@ -10,7 +10,9 @@ Widom run, launched as one batch (binning kernels, one persistent tiled-kernel
 launch, reductions).  Inputs are resident in HBM before the timed region.
 The plain path (one framework per launch, synchronous, grids left resident
 as the batch leaves them) runs the same work after the timed region for the
-batched/plain ratio.
+batched/plain ratio.  The reference's direct method on the same GPU (FP64,
+every (point, image, atom) triple, one framework per launch) is timed on a
+sample of the mix and scaled to the whole mix by pair-evals.
 
 value = pair-evals/s in the reference's enumeration (P*N*I for grids plus
 n_ins*N*I for Widom, the same count for every path).  The roofline figure
@@ -325,6 +327,9 @@ def main():
     ap.add_argument("--frameworks", type=int, default=256)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--direct-every", type=int, default=16,
+                    help="time the reference's direct method (FP64, every triple, one "
+                         "framework per launch) on every k-th framework (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--graph", type=int, default=1, help="also time hipGraph replays")
     ap.add_argument("--ppl", type=int, default=0, help="points per lane (0: library default)")
@@ -460,6 +465,33 @@ def main():
         plain_s = time.perf_counter() - t0
         log(f"plain step {plain_s * 1e3:.1f} ms")
 
+    direct = None
+    if args.direct_every > 0:
+        # the reference's method on the same GPU: FP64 arithmetic over every
+        # (point, image, atom) triple, one framework per launch, grids kept
+        # resident, Widom sums back; timed on a sample of the mix and scaled
+        # to the whole mix by pair-evals (its time is linear in them)
+        k = args.direct_every
+        sample = list(range(0, len(fws), k))
+        sample_pe = sum(sum(pair_evals(fws[i])) for i in sample)
+        t0 = time.perf_counter()
+        for i in sample:
+            f = fws[i]
+            a = f.particles()
+            args_d = (a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2)
+            api.direct_grid(ctx, *args_d, f.grid_n(), download=False).close()
+            api.direct_widom(ctx, *args_d, f.n_insertions, f.temperature, f.seed)
+        t_direct = time.perf_counter() - t0
+        est_s = t_direct * (grid_pe + widom_pe) / sample_pe
+        direct = {"sample": f"every {k}-th framework ({len(sample)} of {len(fws)}), grid + "
+                            "Widom each, scaled to the mix by pair-evals",
+                  "seconds": t_direct, "pair_evals_per_s": sample_pe / t_direct,
+                  "est_seconds_per_step": est_s,
+                  "est_frameworks_per_s": len(fws) / est_s,
+                  "batched_over_direct": est_s / t_batched}
+        log(f"direct method {sample_pe / t_direct:.3g} pair-evals/s on the sample "
+            f"({t_direct:.1f} s); whole mix ~{est_s:.1f} s")
+
     # in-cutoff pairs of one step (exact GPU counting pass, outside the timed region)
     incut = 0
     for f, u in zip(fws, uploaded):
@@ -503,6 +535,7 @@ def main():
         "frameworks_per_s_batched": len(fws) / t_batched,
         "frameworks_per_s_plain": (len(fws) / plain_s) if plain_s else None,
         "batched_over_plain": (plain_s / t_batched) if plain_s else None,
+        "direct_method": direct,
         "graph_ms_per_step": graph_ms,
         "incut_pairs_per_step": incut,
         "kh_rel_err_max": kh_err,

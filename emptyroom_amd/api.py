@@ -138,7 +138,8 @@ EXPORTS = [
     "er_host_run_write_settings", "er_host_run_release", "er_write_grid_file",
     "er_replication_factors", "er_write_uc_file", "er_accessibility_grid",
     "er_grid_file_info", "er_read_grid_file", "er_write_grid_binary", "er_grid_load",
-    "er_tuning_defaults", "er_grid_tile_for", "er_grid_energies",
+    "er_tuning_defaults", "er_grid_tile_for", "er_grid_energies", "er_direct_grid",
+    "er_direct_widom",
 ]
 
 _lib: Optional[ctypes.CDLL] = None
@@ -217,6 +218,12 @@ _SIGS = {
     "er_grid_load": (ctypes.c_int, [_P, ctypes.c_char_p, ctypes.POINTER(_P)]),
     "er_accessibility_grid": (ctypes.c_int, [_P, _P, ctypes.c_int, _P,
                                              ctypes.POINTER(ctypes.c_int), _P]),
+    "er_direct_grid": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_int),
+                                      ctypes.c_double, ctypes.POINTER(ctypes.c_int), _P,
+                                      ctypes.POINTER(_P)]),
+    "er_direct_widom": (ctypes.c_int, [_P, _P, ctypes.c_int, _P, ctypes.POINTER(ctypes.c_int),
+                                       ctypes.c_double, ctypes.c_uint64, ctypes.c_double,
+                                       ctypes.c_uint64, ctypes.POINTER(WidomResult), _P]),
     "er_tuning_defaults": (ctypes.c_int, [ctypes.POINTER(Tuning)]),
     "er_grid_energies": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_double), ctypes.c_uint64,
                                         ctypes.POINTER(ctypes.c_double)]),
@@ -518,6 +525,44 @@ def accessibility_grid(ctx: Context, frac: np.ndarray, eps: np.ndarray, sig2: np
                                        t.ctypes.data_as(ctypes.c_void_p), _iarr(n),
                                        out.ctypes.data_as(ctypes.c_void_p)))
     return out.reshape(int(n[0]), int(n[1]), int(n[2]))
+
+
+def _direct_args(frac, eps, sig2, t_matrix, rep):
+    arr, na = particles_array(np.asarray(frac, dtype=np.float64),
+                              np.asarray(eps, dtype=np.float64),
+                              np.asarray(sig2, dtype=np.float64))
+    t = np.ascontiguousarray(np.asarray(t_matrix, dtype=np.float64).reshape(9))
+    return arr, na, t, _iarr(rep)
+
+
+def direct_grid(ctx: Context, frac: np.ndarray, eps: np.ndarray, sig2: np.ndarray,
+                t_matrix: np.ndarray, rep: Sequence[int], rc2: float, n: Sequence[int],
+                download: bool = True):
+    """The reference's direct method on the GPU (FP64, every triple): the grid
+    as float64 (download=True) or a resident Grid."""
+    arr, na, t, r = _direct_args(frac, eps, sig2, t_matrix, rep)
+    out = np.empty(int(n[0]) * int(n[1]) * int(n[2]), dtype=np.float64) if download else None
+    g = _P()
+    _check(lib().er_direct_grid(ctx.h, ctypes.cast(arr, ctypes.c_void_p), na,
+                                t.ctypes.data_as(ctypes.c_void_p), r, float(rc2), _iarr(n),
+                                out.ctypes.data_as(ctypes.c_void_p) if download else None,
+                                None if download else ctypes.byref(g)))
+    return out.reshape(int(n[0]), int(n[1]), int(n[2])) if download else Grid(g)
+
+
+def direct_widom(ctx: Context, frac: np.ndarray, eps: np.ndarray, sig2: np.ndarray,
+                 t_matrix: np.ndarray, rep: Sequence[int], rc2: float, n: int,
+                 temperature: float, seed: int, energies: bool = False):
+    """Widom sums (sum_b, sum_eb, n) by the direct method at the points of
+    widom(); optionally E per insertion."""
+    arr, na, t, r = _direct_args(frac, eps, sig2, t_matrix, rep)
+    res = WidomResult()
+    e = np.empty(int(n), dtype=np.float64) if energies else None
+    _check(lib().er_direct_widom(ctx.h, ctypes.cast(arr, ctypes.c_void_p), na,
+                                 t.ctypes.data_as(ctypes.c_void_p), r, float(rc2), int(n),
+                                 float(temperature), int(seed), ctypes.byref(res),
+                                 e.ctypes.data_as(ctypes.c_void_p) if energies else None))
+    return (res.sum_b, res.sum_eb, res.n, e) if energies else (res.sum_b, res.sum_eb, res.n)
 
 
 def pockets_label_batch(ctx: Context, grids: Sequence[Grid], temperature: float,

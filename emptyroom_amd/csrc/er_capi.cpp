@@ -626,6 +626,40 @@ int er_accessibility_grid(er_context *ctx, const er_particle *atoms, int n_atoms
     });
 }
 
+int er_direct_grid(er_context *ctx, const er_particle *atoms, int n_atoms,
+                   const double t_matrix[9], const int rep[3], double rc2, const int n[3],
+                   double *host_out, er_grid **resident_out) {
+    if (!ctx || !t_matrix || !rep || !n || n_atoms < 0 || (n_atoms > 0 && !atoms))
+        return arg_error("er_direct_grid: bad arguments");
+    return guarded([&]() -> int {
+        er::GridDev *g = nullptr;
+        int st = er::direct_grid(C(ctx), reinterpret_cast<const er::Particle *>(atoms), n_atoms,
+                                 t_matrix, rep, rc2, n, host_out, resident_out ? &g : nullptr);
+        if (st == ST_OK && resident_out) *resident_out = reinterpret_cast<er_grid *>(g);
+        return st;
+    });
+}
+
+int er_direct_widom(er_context *ctx, const er_particle *atoms, int n_atoms,
+                    const double t_matrix[9], const int rep[3], double rc2,
+                    uint64_t n_insertions, double temperature, uint64_t seed,
+                    er_widom_result *out, double *energies_out) {
+    if (!ctx || !t_matrix || !rep || !out || n_atoms < 0 || (n_atoms > 0 && !atoms))
+        return arg_error("er_direct_widom: bad arguments");
+    return guarded([&]() -> int {
+        er::WidomResult r;
+        int st = er::direct_widom(C(ctx), reinterpret_cast<const er::Particle *>(atoms), n_atoms,
+                                  t_matrix, rep, rc2, n_insertions, temperature, seed, &r,
+                                  energies_out);
+        if (st == ST_OK) {
+            out->sum_b = r.sum_b;
+            out->sum_eb = r.sum_eb;
+            out->n = r.n;
+        }
+        return st;
+    });
+}
+
 int er_grid_file_info(const char *path, int n[3]) {
     if (!path || !n) return arg_error("er_grid_file_info: null argument");
     return guarded([&]() -> int {

@@ -164,6 +164,16 @@ int count_incut_widom(Context *ctx, FrameworkDev *fw, uint64_t n_ins, uint64_t s
 // grid points f = i / (N - 1), layout k + j*nz + i*ny*nz.
 int accessibility_grid(Context *ctx, const Particle *atoms, int n_atoms, const double t[9],
                        const int n[3], double *host_out);
+// The reference's direct method (src/computegridsheet.h:30-99,
+// src/doHenry.h:29-135) on the GPU in FP64, every (point, image, atom)
+// triple, atoms in the caller's order (er_direct.hip).  The grid is FP64;
+// resident (nullable) keeps it on the device.
+int direct_grid(Context *ctx, const Particle *atoms, int n_atoms, const double t[9],
+                const int rep[3], double rc2, const int n[3], double *host_out,
+                GridDev **resident);
+int direct_widom(Context *ctx, const Particle *atoms, int n_atoms, const double t[9],
+                 const int rep[3], double rc2, uint64_t n_ins, double temperature, uint64_t seed,
+                 WidomResult *out, double *energies_host);
 
 // Device buffers and job descriptors of one Widom job.
 struct WidomBuffers {

@@ -72,7 +72,8 @@ typedef struct er_widom_result {
     uint64_t n;
 } er_widom_result;
 
-/* Kernel shape knobs (see DESIGN notes in the repository). */
+/* Shape knobs of the tile kernel (grids and direct Widom).  They decide the
+ * order of every FP64 sum, so a context keeps them for all of its work. */
 typedef struct er_tuning {
     int points_per_lane;   /* 1, 2 or 4 (Widom insertions: at most 2 per lane) */
     int lds_records;       /* LDS candidate-list capacity, 512..4800 records of 32 B */
@@ -98,7 +99,9 @@ const char *er_version(void);
 
 /* ---- context ------------------------------------------------------------ */
 /* hbm_cap_bytes: 0 = whole device; otherwise device allocations of this
- * context (frameworks, grids, scratch) are refused beyond the cap. */
+ * context (frameworks, grids, scratch) are refused beyond the cap.
+ * Frameworks, grids, batches and servers use their context's memory pool
+ * and stream: release them before er_context_destroy. */
 int er_context_create(int device, size_t hbm_cap_bytes, er_context **out);
 int er_context_destroy(er_context *ctx);
 int er_context_get_tuning(const er_context *ctx, er_tuning *out);
@@ -323,6 +326,25 @@ int er_write_grid_file(const char *path, const char *format, const double *e64, 
  * N_i = ceil(1 / resolution) (src/writegrid.cu:104-108). */
 int er_accessibility_grid(er_context *ctx, const er_particle *atoms, int n_atoms,
                           const double t_matrix[9], const int n[3], double *out);
+
+/* ---- the reference's direct method on the GPU -------------------------------
+ * ComputeGridSheet (src/computegridsheet.h:30-99, whole grid in one launch)
+ * and doHenry (src/doHenry.h:29-135, one insertion per lane) with the
+ * reference's arithmetic in FP64 over every (point, image, atom) triple in
+ * its order, atoms in the caller's order: no pruning and no mixed precision.
+ * A baseline for the tile kernel (the reference's method on the same GPU)
+ * and a second device implementation of the reference arithmetic.  Grid:
+ * FP64, layout and points as er_grid_compute, at most 2^31 points;
+ * host_out and resident_out are each nullable.  Widom: the Philox points of
+ * er_widom (same seed, same points), n_insertions in [1, 2^31], sums as
+ * er_widom; energies_out (nullable) receives double[n_insertions]. */
+int er_direct_grid(er_context *ctx, const er_particle *atoms, int n_atoms,
+                   const double t_matrix[9], const int rep[3], double rc2, const int n[3],
+                   double *host_out, er_grid **resident_out);
+int er_direct_widom(er_context *ctx, const er_particle *atoms, int n_atoms,
+                    const double t_matrix[9], const int rep[3], double rc2,
+                    uint64_t n_insertions, double temperature, uint64_t seed,
+                    er_widom_result *out, double *energies_out);
 /* ---- grid files ---------------------------------------------------------
  * Text grids (src/writegrid.cu:144-147,260-269; the format er_write_grid_file
  * and the drop-in writegrid write) are read as src/gcmc.cc:286-346 imports
