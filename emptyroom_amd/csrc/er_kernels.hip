@@ -1024,8 +1024,39 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                 continue;
             }
             // ---- one Widom bin: every insertion whose point falls in it ----
-            // (at most 2 per lane: bins hold a few hundred insertions)
+            // (at most 2 per lane: bins hold a few hundred insertions).  A
+            // chunk's slots are c + q*256 + tid; a wave none of whose lanes
+            // holds a second slot evaluates one point per lane, and a wave
+            // with no slot at all skips the chunk (the energies do not depend
+            // on the points per lane).
             constexpr int PW = PPL > 2 ? 2 : PPL;
+            auto chunk = [&](auto pw, uint64_t c, uint64_t end, const Tile &tl, int nseg,
+                             bool first) {
+                constexpr int P = decltype(pw)::value;
+                double px[P], py[P], pz[P], acc[P];
+                uint32_t ins[P];
+                bool ok[P];
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    const uint64_t p = c + (uint64_t)(q * kWG + tid);
+                    ok[q] = p < end;
+                    ins[q] = J->sorted[ok[q] ? p : c];
+                    double f[3];
+                    widom_point(J->seed, J->stream, ins[q], f);
+                    frac_to_cart(J->t, f[0], f[1], f[2], px[q], py[q], pz[q]);
+                    px[q] -= tl.cx;
+                    py[q] -= tl.cy;
+                    pz[q] -= tl.cz;
+                    acc[q] = 0.0;
+                }
+                eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R);
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    if (!ok[q]) continue;
+                    double *dst = J->e_out + ins[q];
+                    *dst = first ? acc[q] : *dst + acc[q];
+                }
+            };
             const int b = (tc[0] * J->nbins[1] + tc[1]) * J->nbins[2] + tc[2];
             const uint32_t s0 = J->bin_start[b], s1 = J->bin_start[b + 1];
             if (s0 == s1) continue;
@@ -1038,30 +1069,13 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                 const int nseg = fill_list<MATH>(J, src, tbeg, tl, cur_t, cur_pos, cap, s_list,
                                                  s_seg, s_cnt);
                 __syncthreads();
-                for (uint32_t c = s0; c < s1; c += (uint32_t)(PW * kWG)) {
-                    double px[PW], py[PW], pz[PW], acc[PW];
-                    uint32_t ins[PW];
-                    bool ok[PW];
-#pragma unroll
-                    for (int q = 0; q < PW; ++q) {
-                        const uint32_t p = c + (uint32_t)(q * kWG + tid);
-                        ok[q] = p < s1;
-                        ins[q] = J->sorted[ok[q] ? p : s0];
-                        double f[3];
-                        widom_point(J->seed, J->stream, ins[q], f);
-                        frac_to_cart(J->t, f[0], f[1], f[2], px[q], py[q], pz[q]);
-                        px[q] -= tl.cx;
-                        py[q] -= tl.cy;
-                        pz[q] -= tl.cz;
-                        acc[q] = 0.0;
-                    }
-                    eval_list<PW, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R);
-#pragma unroll
-                    for (int q = 0; q < PW; ++q) {
-                        if (!ok[q]) continue;
-                        double *dst = J->e_out + ins[q];
-                        *dst = first ? acc[q] : *dst + acc[q];
-                    }
+                for (uint64_t c = s0; c < s1; c += (uint64_t)(PW * kWG)) {
+                    const uint64_t w0 = c + (uint64_t)(tid & ~63);  // wave-uniform
+                    if (w0 >= s1) continue;
+                    if (PW > 1 && w0 + kWG >= s1)
+                        chunk(std::integral_constant<int, 1>(), c, s1, tl, nseg, first);
+                    else
+                        chunk(std::integral_constant<int, PW>(), c, s1, tl, nseg, first);
                 }
                 first = false;
                 __syncthreads();
