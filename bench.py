@@ -474,23 +474,49 @@ def main():
         k = args.direct_every
         sample = list(range(0, len(fws), k))
         sample_pe = sum(sum(pair_evals(fws[i])) for i in sample)
+        d_grids, d_sums = [], []
         t0 = time.perf_counter()
         for i in sample:
             f = fws[i]
             a = f.particles()
             args_d = (a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2)
-            api.direct_grid(ctx, *args_d, f.grid_n(), download=False).close()
-            api.direct_widom(ctx, *args_d, f.n_insertions, f.temperature, f.seed)
+            d_grids.append(api.direct_grid(ctx, *args_d, f.grid_n(), download=False))
+            d_sums.append(api.direct_widom(ctx, *args_d, f.n_insertions, f.temperature, f.seed))
         t_direct = time.perf_counter() - t0
         est_s = t_direct * (grid_pe + widom_pe) / sample_pe
+        # the batch's own results against the direct method's FP64 ones, at
+        # every grid point and over every insertion of the sampled frameworks
+        gerr, kerr, eerr, n_pts, n_kh = 0.0, 0.0, 0.0, 0, 0
+        for i, gd, (db, deb, _) in zip(sample, d_grids, d_sums):
+            ref = gd.download()
+            gd.close()
+            m = ref < 1e4
+            if m.any():
+                g = batch.grid(i).astype(np.float64)
+                gerr = max(gerr, float(np.max(np.abs(g[m] - ref[m]))))
+                n_pts += int(m.sum())
+            bb, beb, _ = batch.widom(i)
+            if db > 0:
+                kerr = max(kerr, abs(bb - db) / db)
+                eerr = max(eerr, abs(beb / bb - deb / db))
+                n_kh += 1
         direct = {"sample": f"every {k}-th framework ({len(sample)} of {len(fws)}), grid + "
                             "Widom each, scaled to the mix by pair-evals",
                   "seconds": t_direct, "pair_evals_per_s": sample_pe / t_direct,
                   "est_seconds_per_step": est_s,
                   "est_frameworks_per_s": len(fws) / est_s,
-                  "batched_over_direct": est_s / t_batched}
+                  "batched_over_direct": est_s / t_batched,
+                  "batched_errors": {
+                      "grid_max_abs_err_K": gerr if n_pts else None,
+                      "grid_points_compared": n_pts,
+                      "kh_rel_err_max": kerr if n_kh else None,
+                      "e_avg_max_abs_err_K": eerr if n_kh else None,
+                      "widom_runs_compared": n_kh,
+                      "against": "FP64 direct method (bit-identical to the oracle), every grid "
+                                 "point with E < 1e4 K and every insertion of the sample"}}
         log(f"direct method {sample_pe / t_direct:.3g} pair-evals/s on the sample "
-            f"({t_direct:.1f} s); whole mix ~{est_s:.1f} s")
+            f"({t_direct:.1f} s); whole mix ~{est_s:.1f} s; batched vs direct: grid "
+            f"{gerr:.3g} K over {n_pts} points, K_H {kerr:.3g} over {n_kh} runs")
 
     # in-cutoff pairs of one step (exact GPU counting pass, outside the timed region)
     incut = 0

@@ -12,6 +12,12 @@ everywhere, which would leave nothing to compare):
   1e-3 K, every energy is finite or a clamped overlap, and the sums are the
   ones the returned energies give (both sums recomputed on the host in FP64
   agree to 1e-12 of the sum of magnitudes).
+
+Every 32nd framework of the mix is also checked in full against the
+reference's direct method on the GPU (FP64, every triple; bit-identical to
+the oracle at the sizes the oracle runs): every grid point (FP32 within
+1e-3 K where E < 1e4 K, FP64 within 1e-9 relative) and every one of the 1e6
+insertions (energies within 1e-3 K where E < 1e4 K, K_H within 1e-4).
 """
 import numpy as np
 import pytest
@@ -70,3 +76,35 @@ def test_widom_1e7_sample_and_sums(ctx, fw_020):
     assert abs(b.sum() - sb) <= 1e-12 * sb
     eb = np.where(b == 0.0, 0.0, e * b)
     assert abs(eb.sum() - seb) <= 1e-12 * np.abs(eb).sum()
+
+
+@pytest.mark.parametrize("idx", list(range(0, 256, 32)))
+def test_mix_sample_against_direct_method(ctx, idx):
+    f = synth.config3(256)[idx]
+    a = f.particles()
+    n = f.grid_n()
+    print(f"mix framework {f.name}: {len(a)} atoms, grid {n}, spacing {f.spacing}", flush=True)
+    ctx.tuning = api.default_tuning()
+    args = (a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2)
+    fw = api.Framework(ctx, *args)
+    ref = api.direct_grid(ctx, *args, n)
+    g32 = api.compute_grid(ctx, fw, n, api.ER_FP32).astype(np.float64)
+    lo = ref < 1e4
+    if lo.any():
+        assert float(np.max(np.abs(g32[lo] - ref[lo]))) <= 1e-3
+    hi = ~lo & (ref < 3.0e38)
+    if hi.any():
+        assert float(np.max(np.abs(g32[hi] - ref[hi]) / ref[hi])) < 1e-6
+    g64 = api.compute_grid(ctx, fw, n, api.ER_FP64)
+    assert float(np.max(np.abs(g64 - ref) / np.maximum(1.0, np.abs(ref)))) < 1e-9
+    nins, T = f.n_insertions, f.temperature
+    sb, seb, _, e = api.widom(ctx, fw, nins, T, f.seed, energies=True)
+    rb, reb, _, re = api.direct_widom(ctx, *args, nins, T, f.seed, energies=True)
+    fw.close()
+    m = re < 1e4
+    if m.any():
+        assert float(np.max(np.abs(e[m] - re[m]))) <= 1e-3
+    if rb > 0:
+        assert abs(sb - rb) / rb <= 1e-4
+    else:
+        assert sb == 0.0
