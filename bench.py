@@ -200,7 +200,7 @@ def bench_serving(args, api, synth):
         "value": top["sustained_fws_per_s"], "unit": "frameworks/s", "n_gpus": 1,
         "steps": len(rates), "warmup": 0, "ms_per_step": None, "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
-        "dtype": "fp32 pair LJ beyond 2 sigma, fp64 r^2 / close pairs / accumulation",
+        "dtype": "fp32 pair LJ beyond 1.5 sigma, fp64 r^2 / close pairs / accumulation",
         "data": "synthetic (seeded SplitMix64 frameworks, Poisson arrivals)",
         "config": {"workload": "config4: Poisson arrivals of grid / Henry requests, frameworks "
                                "from the batching mix, T ~ U[250,350] K, HBM cap 200 GB",
@@ -550,7 +550,7 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair LJ beyond 2 sigma, fp64 r^2 / close pairs / accumulation",
+        "dtype": "fp32 pair LJ beyond 1.5 sigma, fp64 r^2 / close pairs / accumulation",
         "data": "synthetic (seeded SplitMix64 frameworks)",
         "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload).format(
                        n=len(fws)),

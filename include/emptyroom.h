@@ -43,7 +43,7 @@ typedef enum er_status {
 } er_status;
 
 typedef enum er_precision {
-    ER_FP32 = 0, /* FP32 grid: FP32 LJ for pairs beyond 2 sigma, FP64 r^2, close pairs and sums */
+    ER_FP32 = 0, /* FP32 grid: FP32 LJ for pairs beyond 1.5 sigma (tunable), FP64 r^2, close pairs and sums */
     ER_FP64 = 1  /* FP64 grid: every pair on the FP64 path (the reference's arithmetic) */
 } er_precision;
 
