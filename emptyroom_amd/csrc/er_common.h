@@ -99,6 +99,8 @@ struct JobDev {
     uint32_t stream;
     uint32_t pad2;
     double *e_out;      // FP64 energy per insertion index
+    double sat_e;       // > 0: stop at a partial energy this high (800 T; see k_tiles)
+    double eps_max;     // largest eps_ij of the framework's pair types
     // scheduling
     long long item_begin;  // first global work item of this job
     long long n_items;     // grid tiles or Widom bins of this job in this launch

@@ -515,6 +515,8 @@ __device__ __forceinline__ void pair_r2(const double4 &A, const double (&mx)[PPL
 // (centre relative to the tile centre, radius w), smaller than the tile's;
 // each segment's shell records are then re-classified against it into the
 // wave's own lists of LDS byte offsets wl (see below).
+// phase (wave-uniform): 0 every record, per segment far, shell, near; 1 only
+// the near records; 2 only the far and shell records.
 template <int PPL, int MATH, bool XL>
 __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                                           const double4 *__restrict__ L, int cap,
@@ -523,7 +525,8 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                                           const double (&pz)[PPL], double (&acc)[PPL],
                                           double R, bool ws = false,
                                           double4 wsph = make_double4(0.0, 0.0, 0.0, 0.0),
-                                          uint16_t *__restrict__ wl = nullptr) {
+                                          uint16_t *__restrict__ wl = nullptr, int phase = 0) {
+    const bool do_fs = phase != 1, do_near = phase != 2;
     const double rc2 = J->rc2, min_r2 = J->min_r2;
     const int capA = near_region_start<MATH>(cap);
     const double4 *__restrict__ Ls = L + capA - 1;  // shell k at Ls[-k]
@@ -555,6 +558,7 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             continue;
         }
         if (MATH == MATH_COUNT) {
+            if (!do_fs) continue;  // counting lists hold no near records
 #pragma unroll
             for (int q = 0; q < PPL; ++q) acc[q] += (double)(fe - fb);
             for (int k = sb; k < se; ++k) {
@@ -592,7 +596,7 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                 }
             };
             // far records [fb, fe), then shell records [sb, se), 8 pairs per flush
-            for (int r = fb; r < fe; r += 8) {
+            for (int r = fb; do_fs && r < fe; r += 8) {
                 float a3[PPL], a6[PPL];
 #pragma unroll
                 for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
@@ -606,7 +610,8 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                 for (int q = 0; q < PPL; ++q)
                     acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
             }
-            if (ws) {
+            if (!do_fs) {
+            } else if (ws) {
                 // Shell records beyond r_c + w of the wave's sphere are out of
                 // the cutoff for all of the wave's points, and those within
                 // r_c - w are inside it for all of them (no cutoff test); the
@@ -685,6 +690,7 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                         acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
                 }
             }
+            if (!do_near) continue;
             // Near pairs of an FP32 grid or Widom run: the expanded FP64 r^2 of
             // the far pairs (|a| <= r_FP64 + R keeps it exact to ~1e-14 A^2),
             // the reference's clamp and strict cutoff, u = 1/r^2 from the FP32
@@ -1028,12 +1034,22 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             // chunk's slots are c + q*256 + tid; a wave none of whose lanes
             // holds a second slot evaluates one point per lane, and a wave
             // with no slot at all skips the chunk (the energies do not depend
-            // on the points per lane).
+            // on the points per lane).  Each pass adds its near records, then
+            // its far and shell records, to the insertion's energy.
+            //
+            // Saturation (J->sat_e > 0, jobs whose energies only feed the
+            // Boltzmann sums): no pair energy is below -eps_ij, so once an
+            // insertion's partial energy plus -eps_ij for every record not
+            // yet evaluated reaches sat_e = 800 T, its exact energy is at
+            // least 800 T as well, exp(-E/T) is exactly 0 in FP64 for both,
+            // and neither sum changes.  A wave whose valid insertions are all
+            // saturated stops evaluating them; the stored partial energy
+            // (>= 800 T) then stands in for the exact one.
             constexpr int PW = PPL > 2 ? 2 : PPL;
             auto chunk = [&](auto pw, uint64_t c, uint64_t end, const Tile &tl, int nseg,
-                             bool first) {
+                             bool first, double lb_near, double lb_fs, double lb_later) {
                 constexpr int P = decltype(pw)::value;
-                double px[P], py[P], pz[P], acc[P];
+                double px[P], py[P], pz[P], acc[P], prev[P];
                 uint32_t ins[P];
                 bool ok[P];
 #pragma unroll
@@ -1048,14 +1064,26 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                     py[q] -= tl.cy;
                     pz[q] -= tl.cz;
                     acc[q] = 0.0;
+                    prev[q] = first ? 0.0 : J->e_out[ins[q]];
                 }
-                eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R);
+                const bool sat = J->sat_e > 0.0;
+                auto saturated = [&](double lb) {
+                    bool all = true;
 #pragma unroll
-                for (int q = 0; q < P; ++q) {
-                    if (!ok[q]) continue;
-                    double *dst = J->e_out + ins[q];
-                    *dst = first ? acc[q] : *dst + acc[q];
-                }
+                    for (int q = 0; q < P; ++q)
+                        if (ok[q]) all = all && prev[q] + acc[q] + lb >= J->sat_e;
+                    return __all(all) != 0;
+                };
+                if (sat && !first && saturated(lb_near + lb_fs + lb_later)) return;
+                eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
+                                          false, make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
+                if (!(sat && saturated(lb_fs + lb_later)))
+                    eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc,
+                                              tl.R, false, make_double4(0.0, 0.0, 0.0, 0.0),
+                                              nullptr, 2);
+#pragma unroll
+                for (int q = 0; q < P; ++q)
+                    if (ok[q]) J->e_out[ins[q]] = first ? acc[q] : prev[q] + acc[q];
             };
             const int b = (tc[0] * J->nbins[1] + tc[1]) * J->nbins[2] + tc[2];
             const uint32_t s0 = J->bin_start[b], s1 = J->bin_start[b + 1];
@@ -1069,13 +1097,29 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                 const int nseg = fill_list<MATH>(J, src, tbeg, tl, cur_t, cur_pos, cap, s_list,
                                                  s_seg, s_cnt);
                 __syncthreads();
+                // -eps_ij for every record of this pass (near; far and shell)
+                // and every source record after it
+                double lb_near = 0.0, lb_fs = 0.0, lb_later = 0.0;
+                if (J->sat_e > 0.0) {
+                    for (int sg = 0; sg < nseg; ++sg) {
+                        const int *g = s_seg + kSegInts * sg;
+                        const double eps = 0.25 * J->types[uniform(g[0])].eps4;
+                        lb_fs -= eps * (double)(uniform(g[2]) - uniform(g[1]) + uniform(g[4]) -
+                                                uniform(g[3]));
+                        lb_near -= eps * (double)(uniform(g[6]) - uniform(g[5]));
+                    }
+                    if (cur_t < J->ntypes)
+                        lb_later = -J->eps_max * (double)(uniform(tbeg[J->ntypes]) - cur_pos);
+                }
                 for (uint64_t c = s0; c < s1; c += (uint64_t)(PW * kWG)) {
                     const uint64_t w0 = c + (uint64_t)(tid & ~63);  // wave-uniform
                     if (w0 >= s1) continue;
                     if (PW > 1 && w0 + kWG >= s1)
-                        chunk(std::integral_constant<int, 1>(), c, s1, tl, nseg, first);
+                        chunk(std::integral_constant<int, 1>(), c, s1, tl, nseg, first, lb_near,
+                              lb_fs, lb_later);
                     else
-                        chunk(std::integral_constant<int, PW>(), c, s1, tl, nseg, first);
+                        chunk(std::integral_constant<int, PW>(), c, s1, tl, nseg, first, lb_near,
+                              lb_fs, lb_later);
                 }
                 first = false;
                 __syncthreads();
@@ -1094,11 +1138,9 @@ __device__ __forceinline__ double wave_sum(double v) {
     return v;
 }
 
-// Block g sums the Boltzmann factors b = exp(-E/T) (src/doHenry.h:107) and
-// E*b over the index range [g*n/G, (g+1)*n/G) in a fixed order.
-// Block g of G sums exp(-E/T) and E exp(-E/T) over the contiguous insertion
-// range [g n / G, (g+1) n / G) in a fixed order (thread stride, then a fixed
-// shuffle tree and the four wave sums in order).
+// Block g of G sums exp(-E/T) (src/doHenry.h:107) and E exp(-E/T) over the
+// contiguous insertion range [g n / G, (g+1) n / G) in a fixed order (thread
+// stride, then a fixed shuffle tree and the four wave sums in order).
 __device__ __forceinline__ void widom_partial_block(const double *e, long long n,
                                                    double temperature, long long g, long long G,
                                                    double2 *partials, double *s_b,

@@ -553,7 +553,8 @@ JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void
 }
 
 JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
-                      const WidomBuffers &wb, const Tuning &tu, long long min_items) {
+                      const WidomBuffers &wb, const Tuning &tu, long long min_items,
+                      double temperature) {
     JobDev j;
     std::memset(&j, 0, sizeof(j));
     std::memcpy(j.t, fw->t, sizeof(j.t));
@@ -572,6 +573,15 @@ JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
     j.seed = seed;
     j.stream = 0;
     j.e_out = wb.e;
+    // saturation bound: every pair energy is >= -eps_ij when no eps_ij is
+    // negative (exp(-800) is exactly 0 in FP64)
+    bool eps_ok = true;
+    for (const TypeParams &tp : fw->types) {
+        eps_ok = eps_ok && tp.eps4 >= 0.0;
+        j.eps_max = std::max(j.eps_max, 0.25 * tp.eps4);
+    }
+    j.sat_e = (temperature > 0.0 && std::isfinite(temperature) && eps_ok) ? 800.0 * temperature
+                                                                         : 0.0;
     // regions of bins share one candidate sub-list, like regions of grid tiles
     double frac[3];
     for (int d = 0; d < 3; d++) frac[d] = 1.0 / wb.nbins[d];
@@ -797,7 +807,10 @@ int widom_run(Context *ctx, FrameworkDev *fw, uint64_t n_ins, double temperature
     Scoped tmp{ctx, {}};
     WidomBuffers wb;
     ST_TRY(widom_prepare(ctx, fw, n_ins, &wb, &tmp.ptrs));
-    JobDev job = make_widom_job(fw, n_ins, seed, wb, ctx->tuning, kMinGridItems);
+    // energies returned to the caller are computed in full; otherwise only the
+    // sums are, and saturated insertions may stop early
+    JobDev job = make_widom_job(fw, n_ins, seed, wb, ctx->tuning, kMinGridItems,
+                                energies_host ? 0.0 : temperature);
     JobDev *d_job = nullptr;
     ST_TRY(pool_alloc(ctx, 1, &d_job, &tmp.ptrs));
     hipStream_t s = ctx->stream;
@@ -1030,7 +1043,8 @@ int batch_create(Context *ctx, const Task *tasks, int n, bool use_graph, Batch *
             // many jobs share the launch: a Widom job needs only its share of
             // the device's work items
             JobDev j = make_widom_job(tk.fw, tk.n_ins, tk.seed, b->wbufs[t], ctx->tuning,
-                                      std::max<long long>(1, kMinGridItems / std::max(n, 1)));
+                                      std::max<long long>(1, kMinGridItems / std::max(n, 1)),
+                                      tk.temperature);
             mixed.push_back({job_cost_per_item(tk.fw, j, ctx->tuning), j});
         }
     }

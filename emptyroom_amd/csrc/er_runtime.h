@@ -193,9 +193,13 @@ JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void
 void choose_region(const FrameworkDev *fw, const double cell_frac[3], const int ncell[3],
                    const Tuning &tu, long long min_items, int region[3]);
 // min_items: regions of bins are shrunk until the job has at least this many
-// work items (kMinGridItems for a job launched alone).
+// work items (kMinGridItems for a job launched alone).  temperature > 0 lets
+// the tile kernel stop an insertion once its energy is certain to reach
+// 800 T (for jobs whose energies only feed the Boltzmann sums); 0 computes
+// every energy in full.
 JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
-                      const WidomBuffers &wb, const Tuning &tu, long long min_items);
+                      const WidomBuffers &wb, const Tuning &tu, long long min_items,
+                      double temperature = 0.0);
 // Allocates the Widom buffers of a job (owned pointers appended to `owned`).
 int widom_prepare(Context *ctx, FrameworkDev *fw, uint64_t n_ins, WidomBuffers *wb,
                   std::vector<void *> *owned);

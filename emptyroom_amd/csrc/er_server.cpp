@@ -164,7 +164,7 @@ bool Server::admit_one(Req *r) {
         st = widom_prepare(ctx_, r->fw, s.n_ins, &r->wb, &r->owned);
         if (st == ST_OK) {
             r->widom_job = make_widom_job(r->fw, s.n_ins, s.seed, r->wb, ctx_->tuning,
-                                          kMinGridItems);
+                                          kMinGridItems, s.temperature);
             r->items_total += r->widom_job.n_items;
             st = widom_enqueue_binning(ctx_, r->wb, s.n_ins, s.seed, 0u, ctx_->stream);
         }

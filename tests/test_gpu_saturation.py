@@ -1,0 +1,42 @@
+"""Widom saturation: when only the Boltzmann sums are wanted, the tile kernel
+stops evaluating an insertion once its partial energy plus -eps_ij for every
+record not yet evaluated reaches 800 T (then exp(-E/T) is exactly 0 for the
+partial and the exact energy alike).  The sums must be bit-identical to a
+run that computes every energy in full (energies requested), and agree with
+the direct method (FP64, every triple) to 1e-4.  Dense frameworks of the
+batching mix, where most insertions overlap atoms, exercise the early stop.
+"""
+import numpy as np
+import pytest
+
+from emptyroom_amd import api, synth
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("idx", [8, 24, 72, 192])
+def test_saturated_sums_bit_identical(ctx, idx):
+    f = synth.config3(256)[idx]
+    a = f.particles()
+    print(f"saturation {f.name}: {len(a)} atoms", flush=True)
+    ctx.tuning = api.default_tuning()
+    args = (a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2)
+    fw = api.Framework(ctx, *args)
+    nins, T = 200000, f.temperature
+    sb, seb, n = api.widom(ctx, fw, nins, T, f.seed)                  # may stop early
+    fb, feb, _, e = api.widom(ctx, fw, nins, T, f.seed, energies=True)  # every energy in full
+    assert n == nins
+    assert (sb, seb) == (fb, feb)
+    b = api.Batch(ctx, [dict(fw=fw, grid_n=[0, 0, 0], precision=api.ER_FP32, n_insertions=nins,
+                             temperature=T, seed=f.seed)], use_graph=False)
+    b.run()
+    b.wait()
+    assert b.widom(0)[:2] == (sb, seb)
+    b.close()
+    fw.close()
+    assert np.sum(e >= 800.0 * T) > 0  # some insertions are saturated
+    rb, reb, _ = api.direct_widom(ctx, *args, nins, T, f.seed)
+    if rb > 0:
+        assert abs(sb - rb) / rb <= 1e-4
+    else:
+        assert sb == 0.0
