@@ -148,7 +148,10 @@ int er_grid_release(er_grid *grid);
  * keyed by `seed` (the reference draws from cuRAND MTGP32 seeded 1234),
  * guest-framework energy with the r^2 clamp 1e-32, FP64 exp(-E/T) and FP64
  * sums in a fixed order.  energies_out (nullable) receives E (K) per
- * insertion index.  K_H and <E> follow from er_henry_result. */
+ * insertion index, every energy computed in full.  Without it, an insertion
+ * whose energy is certain to reach 800 T (no pair energy is below -eps_ij)
+ * stops early: its exp(-E/T) is exactly 0 either way, so the sums are
+ * bit-identical.  K_H and <E> follow from er_henry_result. */
 int er_widom(er_context *ctx, er_framework *fw, uint64_t n_insertions, double temperature,
              uint64_t seed, er_widom_result *out, double *energies_out);
 
