@@ -40,3 +40,23 @@ def test_saturated_sums_bit_identical(ctx, idx):
         assert abs(sb - rb) / rb <= 1e-4
     else:
         assert sb == 0.0
+
+
+@pytest.mark.parametrize("ppl", [1, 4])
+def test_saturated_sums_other_points_per_lane(ctx, ppl):
+    """The stop is per wave, so its decisions change with the points per lane;
+    the sums must not."""
+    f = synth.config3(256)[24]
+    a = f.particles()
+    print(f"saturation {f.name}, {ppl} points per lane", flush=True)
+    tu = api.default_tuning()
+    tu.points_per_lane = ppl
+    tu.tile[0], tu.tile[1], tu.tile[2] = (8, 8, 4) if ppl == 1 else (16, 8, 8)
+    ctx.tuning = tu
+    fw = api.Framework(ctx, a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2)
+    nins, T = 100000, f.temperature
+    sb, seb, _ = api.widom(ctx, fw, nins, T, f.seed)
+    fb, feb, _, _ = api.widom(ctx, fw, nins, T, f.seed, energies=True)
+    fw.close()
+    ctx.tuning = api.default_tuning()
+    assert (sb, seb) == (fb, feb)
