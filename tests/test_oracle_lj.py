@@ -196,7 +196,7 @@ def test_widom_energies_are_energies_at_the_stream_points():
 def test_henry_counts_reference_answers():
     # IRMOF-1, 1000 insertions per A^3, values printed by the reference's host code
     assert oracle.henry_counts(1000, 17237.492730368002) == (17237492, 1053, 17252352)
-    # survey example: 1e5 requested -> 16384 * 7 = 114688 executed
+    # 1e5 requested -> 16384 * (1e5 / 16384 + 1) = 114688 executed (src/henry.cu:94-95)
     ni, pt, tot = oracle.henry_counts(1, 100000.5)
     assert (ni, pt, tot) == (100000, 7, 114688)
 
