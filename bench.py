@@ -327,6 +327,9 @@ def main():
     ap.add_argument("--frameworks", type=int, default=256)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--grid-precision", default="fp32", choices=["fp32", "fp64"],
+                    help="grids of the batch: fp32 (mixed precision) or fp64 (every pair "
+                         "FP64, the drop-in writegrid's default)")
     ap.add_argument("--direct-every", type=int, default=16,
                     help="time the reference's direct method (FP64, every triple, one "
                          "framework per launch) on every k-th framework (0: skip)")
@@ -393,7 +396,8 @@ def main():
     for f in fws:
         a = f.particles()
         uploaded.append(api.Framework(ctx, a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2))
-    tasks = [dict(fw=u, grid_n=f.grid_n(), precision=api.ER_FP32, n_insertions=f.n_insertions,
+    gprec = api.ER_FP64 if args.grid_precision == "fp64" else api.ER_FP32
+    tasks = [dict(fw=u, grid_n=f.grid_n(), precision=gprec, n_insertions=f.n_insertions,
                   temperature=f.temperature, seed=f.seed) for f, u in zip(fws, uploaded)]
     tuned = None
     if args.autotune:
@@ -433,8 +437,8 @@ def main():
     for _ in range(args.steps):
         batch.run()
         batch.wait()
-        ms, nl = batch.kernel_ms()
-        kernel_ms.append(ms / max(nl, 1))
+        ms, nl = batch.kernel_ms()  # every tile-kernel launch of the step
+        kernel_ms.append(ms)
     t_batched = (time.perf_counter() - t0) / args.steps
     log(f"batched step {t_batched * 1e3:.1f} ms, tiled kernel {np.mean(kernel_ms):.1f} ms")
 
@@ -460,7 +464,7 @@ def main():
         # launch, synchronous
         t0 = time.perf_counter()
         for f, u in zip(fws, uploaded):
-            api.Grid.compute(ctx, u, f.grid_n(), api.ER_FP32).close()
+            api.Grid.compute(ctx, u, f.grid_n(), gprec).close()
             api.widom(ctx, u, f.n_insertions, f.temperature, f.seed)
         plain_s = time.perf_counter() - t0
         log(f"plain step {plain_s * 1e3:.1f} ms")
@@ -556,12 +560,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 pair LJ beyond 1.5 sigma, fp64 r^2 / close pairs / accumulation",
+        "dtype": ("fp32 pair LJ beyond 1.5 sigma, fp64 r^2 / close pairs / accumulation"
+                  if gprec == api.ER_FP32 else
+                  "fp64 grids (every pair), Widom fp32 pair LJ beyond 1.5 sigma"),
         "data": "synthetic (seeded SplitMix64 frameworks)",
         "config": {"workload": WORKLOAD_TEXT.get(args.workload, args.workload).format(
-                       n=len(fws)),
+                       n=len(fws)).replace("FP32", args.grid_precision.upper()),
                    "frameworks": len(fws), "grid_points": points, "insertions": n_ins,
-                   "parallelism": "single GPU"},
+                   "grid_precision": args.grid_precision, "parallelism": "single GPU"},
         "inserts_per_s": n_ins / t_batched,
         "grid_pair_evals_per_s": grid_pe / t_batched,
         "frameworks_per_s_batched": len(fws) / t_batched,
@@ -578,8 +584,11 @@ def main():
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
                      "traffic": measured_traffic(tuning, len(fws))
-                     if args.workload == "config3" else None,
-                     "kernel": f"k_tiles<{tuning['ppl']},MIXED>", "kernel_ms": kern_s * 1e3,
+                     if args.workload == "config3" and gprec == api.ER_FP32 else None,
+                     "kernel": f"k_tiles<{tuning['ppl']},MIXED>" if gprec == api.ER_FP32 else
+                               f"k_tiles<{tuning['ppl']},FP64> (grids) + "
+                               f"k_tiles<{tuning['ppl']},MIXED> (Widom)",
+                     "kernel_ms": kern_s * 1e3,
                      "flop_per_pair": FLOP_PER_PAIR,
                      "note": "16 FLOP x every in-cutoff pair of the step, including the Widom "
                              "pairs of insertions stopped once their energy is certain to "
