@@ -507,10 +507,10 @@ __device__ __forceinline__ void pair_r2(const double4 &A, const double (&mx)[PPL
 //   4 eps (sigma^12 S6 - sigma^6 S3).  sigma and eps stay FP64, so their
 //   rounding adds no error common to every pair of a type.
 //   Shell pairs: the same, behind the strict FP64 test r^2 < r_c^2.
-//   Near pairs: FP64 displacement and r^2, the reference's clamp and strict
-//   cutoff test, FP64 LJ with a Newton-refined reciprocal.
-// XL: see pair_r2 (never used for FP64 grids, whose near pairs use
-// displacements).
+//   Near pairs (every pair of an FP64 grid): the same FP64 r^2, the
+//   reference's clamp and strict cutoff test, FP64 LJ with a Newton-refined
+//   reciprocal (one step for FP32 grids and Widom, two for FP64 grids).
+// XL: see pair_r2.
 // ws (MIXED only, wave-uniform): this wave's points lie in the sphere wsph
 // (centre relative to the tile centre, radius w), smaller than the tile's;
 // each segment's shell records are then re-classified against it into the
@@ -727,24 +727,33 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             for (int q = 0; q < PPL; ++q) acc[q] = fma(c6, b6[q], fma(c3, b3[q], acc[q]));
             continue;
         }
-        // FP64 grids: the reference's arithmetic on displacements
-        const double sig2 = P.sig2, eps4 = P.eps4;
+        // FP64 grids: every record is near.  The expanded FP64 r^2 of the
+        // near pairs of FP32 grids, the reference's clamp and strict cutoff,
+        // u = 1/r^2 from the FP32 reciprocal and two Newton steps (FP64
+        // accuracy), FP64 sums of u^3 and u^6 scaled by 4 eps sigma^6 /
+        // sigma^12 per segment.
+        const double sig6 = P.sig2 * P.sig2 * P.sig2;
+        const double c6 = P.eps4 * sig6 * sig6, c3 = -P.eps4 * sig6;
+        double b3[PPL], b6[PPL];
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) b3[q] = b6[q] = 0.0;
         for (int k = nb; k < ne; ++k) {
-            const double4 A = Ln[k];
+            double r2[PPL];
+            pair_r2<PPL, XL>(Ln[k], mx, my, mz, p2, r2);
 #pragma unroll
             for (int q = 0; q < PPL; ++q) {
-                const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
-                const double r2 = fmax(fma(dx, dx, fma(dy, dy, dz * dz)), min_r2);  // :77
-                const bool in = r2 < rc2;  // src/computegridsheet.h:79
-                double y = (double)__builtin_amdgcn_rcpf((float)r2);
-                y = fma(fma(-r2, y, 1.0), y, y);
-                y = fma(fma(-r2, y, 1.0), y, y);
-                const double s2 = sig2 * y;
-                const double s6 = s2 * s2 * s2;
-                const double e = eps4 * fma(s6, s6, -s6);
-                acc[q] += in ? e : 0.0;
+                const double rr = fmax(r2[q], min_r2);  // src/computegridsheet.h:77
+                double y = (double)__builtin_amdgcn_rcpf((float)rr);
+                y = fma(fma(-rr, y, 1.0), y, y);
+                y = fma(fma(-rr, y, 1.0), y, y);
+                double y3 = y * y * y;
+                y3 = rr < rc2 ? y3 : 0.0;  // src/computegridsheet.h:79
+                b3[q] += y3;
+                b6[q] = fma(y3, y3, b6[q]);
             }
         }
+#pragma unroll
+        for (int q = 0; q < PPL; ++q) acc[q] = fma(c6, b6[q], fma(c3, b3[q], acc[q]));
     }
 }
 
@@ -811,8 +820,8 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
     // A lane's points differ only in i (quadrant layout, or p = tid + 256 q
     // with t1 * t2 dividing 256), so on a cell whose first vector lies along
     // x (t_yx = t_zx = 0, as cell parameters give it) only in x.
-    const bool xl = MATH != MATH_FP64 && PPL > 1 && (quad || kWG % (t1 * t2) == 0) &&
-                    J->t[3] == 0.0 && J->t[6] == 0.0;
+    const bool xl = PPL > 1 && (quad || kWG % (t1 * t2) == 0) && J->t[3] == 0.0 &&
+                    J->t[6] == 0.0;
     // The sphere around this wave's valid points (quadrant layout): its
     // index box is the tile's i range times the quadrant's j and k ranges.
     bool ws = false, idle = false;
