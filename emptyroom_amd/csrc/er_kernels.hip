@@ -324,7 +324,7 @@ __device__ __forceinline__ Tile tile_sphere(const double *t, const double lo[3],
 // shell records (from capA-1 downwards); [capA, cap) holds near records.
 template <int MATH>
 __device__ __forceinline__ int near_region_start(int cap) {
-    if (MATH == MATH_FP64) return kWG;        // every record is near
+    if (MATH == MATH_FP64) return cap - kWG;  // near: atoms within R of the tile centre
     if (MATH == MATH_COUNT || MATH == MATH_OVERLAP) return cap - kWG;  // no near records
     return mixed_far_shell_cap(cap);
 }
@@ -356,6 +356,9 @@ __device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict
     const double in2 = (rc + tl.R + kMargin) * (rc + tl.R + kMargin);
     const double far_r = rc - tl.R - kMargin;
     const double far2 = far_r > 0.0 ? far_r * far_r : -1.0;
+    // FP64 grids: near (clamp possible) only if some point may lie within
+    // kMargin of the atom, d <= R + kMargin
+    const double clamp2 = (tl.R + kMargin) * (tl.R + kMargin);
     double near2 = -1.0;
     int near_t = -1;
     int nf = 0, ns = 0, nn = 0, nseg = 0, f0 = 0, s0 = 0, n0 = 0;
@@ -391,7 +394,7 @@ __device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict
             const double d2 = dx * dx + dy * dy + dz * dz;
             A = make_double4(dx, dy, dz, d2);
             if (d2 < in2) {
-                if (MATH == MATH_FP64) cat = CAT_NEAR;
+                if (MATH == MATH_FP64 && d2 <= clamp2) cat = CAT_NEAR;
                 else if (MATH == MATH_MIXED && d2 <= near2) cat = CAT_NEAR;
                 else cat = (MATH != MATH_OVERLAP && d2 < far2) ? CAT_FAR : CAT_SHELL;
             }
@@ -727,31 +730,38 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             for (int q = 0; q < PPL; ++q) acc[q] = fma(c6, b6[q], fma(c3, b3[q], acc[q]));
             continue;
         }
-        // FP64 grids: every record is near.  The expanded FP64 r^2 of the
-        // near pairs of FP32 grids, the reference's clamp and strict cutoff,
-        // u = 1/r^2 from the FP32 reciprocal and two Newton steps (FP64
-        // accuracy), FP64 sums of u^3 and u^6 scaled by 4 eps sigma^6 /
-        // sigma^12 per segment.
+        // FP64 grids: every pair in FP64.  The expanded FP64 r^2 of the near
+        // pairs of FP32 grids, u = 1/r^2 from the FP32 reciprocal and two
+        // Newton steps (FP64 accuracy), FP64 sums of u^3 and u^6 scaled by
+        // 4 eps sigma^6 / sigma^12 per segment.  Far records (inside r_c for
+        // every point, farther than kMargin from all) need neither the clamp
+        // nor the cutoff test, shell records only the test, near records
+        // (an atom within R of the tile centre) both.
         const double sig6 = P.sig2 * P.sig2 * P.sig2;
         const double c6 = P.eps4 * sig6 * sig6, c3 = -P.eps4 * sig6;
         double b3[PPL], b6[PPL];
 #pragma unroll
         for (int q = 0; q < PPL; ++q) b3[q] = b6[q] = 0.0;
-        for (int k = nb; k < ne; ++k) {
+        auto fp64_pair = [&](const double4 &A, auto clamp, auto test) {
             double r2[PPL];
-            pair_r2<PPL, XL>(Ln[k], mx, my, mz, p2, r2);
+            pair_r2<PPL, XL>(A, mx, my, mz, p2, r2);
 #pragma unroll
             for (int q = 0; q < PPL; ++q) {
-                const double rr = fmax(r2[q], min_r2);  // src/computegridsheet.h:77
+                const double rr = decltype(clamp)::value ? fmax(r2[q], min_r2) : r2[q];  // :77
                 double y = (double)__builtin_amdgcn_rcpf((float)rr);
                 y = fma(fma(-rr, y, 1.0), y, y);
                 y = fma(fma(-rr, y, 1.0), y, y);
                 double y3 = y * y * y;
-                y3 = rr < rc2 ? y3 : 0.0;  // src/computegridsheet.h:79
+                if (decltype(test)::value) y3 = rr < rc2 ? y3 : 0.0;  // src/computegridsheet.h:79
                 b3[q] += y3;
                 b6[q] = fma(y3, y3, b6[q]);
             }
-        }
+        };
+        using no = std::integral_constant<bool, false>;
+        using yes = std::integral_constant<bool, true>;
+        for (int k = fb; k < fe; ++k) fp64_pair(L[k], no(), no());
+        for (int k = sb; k < se; ++k) fp64_pair(Ls[-k], no(), yes());
+        for (int k = nb; k < ne; ++k) fp64_pair(Ln[k], yes(), yes());
 #pragma unroll
         for (int q = 0; q < PPL; ++q) acc[q] = fma(c6, b6[q], fma(c3, b3[q], acc[q]));
     }
