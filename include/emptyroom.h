@@ -44,7 +44,7 @@ typedef enum er_status {
 
 typedef enum er_precision {
     ER_FP32 = 0, /* FP32 grid: FP32 LJ for pairs beyond 1.5 sigma (tunable), FP64 r^2, close pairs and sums */
-    ER_FP64 = 1  /* FP64 grid: every pair on the FP64 path (the reference's arithmetic) */
+    ER_FP64 = 1  /* FP64 grid: every pair in FP64 (the reference's result to ~1e-13 relative) */
 } er_precision;
 
 typedef struct er_context er_context;
