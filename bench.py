@@ -8,16 +8,26 @@ cell lengths U[8,40] A, angles U[60,120] deg, grid spacing in {0.1,0.2,0.5} A
 A step is one pass of the hot path over the whole batch: every grid and every
 Widom run, launched as one batch (binning kernels, one persistent tiled-kernel
 launch, reductions).  Inputs are resident in HBM before the timed region.
-The plain path (one framework per launch, synchronous, grids left resident
-as the batch leaves them) runs the same work after the timed region for the
-batched/plain ratio.  The reference's direct method on the same GPU (FP64,
-every (point, image, atom) triple, one framework per launch) is timed on a
-sample of the mix and scaled to the whole mix by pair-evals.
+After the timed steps, the default run also measures:
+- the plain path (one framework per launch, synchronous, grids left
+  resident as the batch leaves them) on the same work: batched/plain, against
+  the workload's 5x target;
+- the tiled kernel on the step's grids alone (the roofline launch: every
+  in-cutoff grid pair is evaluated, 16 FLOP each, VALU-bound) and on its Widom
+  runs alone (an insertion rate: saturated insertions stop early);
+- the same step with FP64 grids (the drop-in writegrid's default precision)
+  and the FP64 grids alone against the FP64 vector peak;
+- the reference's direct method on the same GPU (FP64, every (point, image,
+  atom) triple, one framework per launch) on a sample of the mix, scaled to
+  the whole mix by pair-evals, with the batch's errors against it;
+- the FP64 oracle on the host (all the cores this job may use, and one) on
+  a bounded sample of the mix, and the plain-method config in full;
+- a serving leg: Poisson arrivals at 500 requests/s for 2 s into the
+  continuous-batching server (p50/p99 latency, sustained requests/s,
+  rejections, framework uploads avoided by residency).
 
 value = pair-evals/s in the reference's enumeration (P*N*I for grids plus
-n_ins*N*I for Widom, the same count for every path).  The roofline figure
-counts 16 FLOP per in-cutoff pair (counted exactly by a GPU counting pass).
-The CPU baseline times the FP64 oracle on a bounded sample of the same work.
+n_ins*N*I for Widom, the same count for every path).
 """
 import argparse
 import json
@@ -31,7 +41,10 @@ import numpy as np
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
-PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector/matrix peak (packed FP32)
+PEAK_FP32_TFLOPS = 157.3  # MI355X FP32 vector peak (packed FP32)
+PEAK_FP32_UNPACKED_TFLOPS = 78.6  # without packed FP32, as this build compiles
+PEAK_FP64_TFLOPS = 78.6  # MI355X FP64 vector peak
+TARGET_BATCHED_OVER_PLAIN = 5.0
 DEFAULT_TUNING_FILE = os.path.join(ROOT, "emptyroom_amd", "tuning", "mi355x_config3.json")
 WORKLOAD_TEXT = {
     "config3": "config3: {n} frameworks, grids FP32 at 0.1/0.2/0.5 A + 1e6 Widom insertions "
@@ -67,13 +80,35 @@ def pair_evals(f):
     return grid, widom
 
 
-def cpu_baseline(fws, uploaded, ctx, batch, budget_s, threads):
+def host_cpu():
+    """Host CPU facts for the baseline: logical CPUs, the ones this process may
+    use, the thread budget of the job (OMP_NUM_THREADS) and the model name."""
+    model = None
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    try:
+        usable = len(os.sched_getaffinity(0))
+    except (AttributeError, OSError):
+        usable = os.cpu_count() or 1
+    omp = os.environ.get("OMP_NUM_THREADS", "")
+    budget = int(omp) if omp.isdigit() and int(omp) > 0 else usable
+    return dict(nproc=os.cpu_count(), usable=usable, omp_num_threads=omp or None,
+                threads=max(1, min(usable, budget)), model=model)
+
+
+def cpu_baseline(fws, uploaded, ctx, batch, budget_s, threads, every=16):
     """Time the oracle on a bounded sample; also report GPU errors on it."""
     from emptyroom_amd import api
     from oracle import oracle
-    # sample: frameworks spread over the mix, a few x-sheets and the first
-    # insertions of each, sized by a quick calibration of the oracle's rate
-    idx = list(range(0, len(fws), max(1, len(fws) // 8)))[:8]
+    # sample: every `every`-th framework of the mix, a few x-sheets and the
+    # first insertions of each, sized by a quick calibration of the oracle's rate
+    idx = list(range(0, len(fws), max(1, every)))
     f0 = fws[idx[0]]
     a0 = f0.particles()
     t0 = time.perf_counter()
@@ -123,74 +158,125 @@ def cpu_baseline(fws, uploaded, ctx, batch, budget_s, threads):
         grid_err = None
     if n_kh_cmp == 0:
         kh_err = e_err = None
-    sample = (f"{len(idx)} of {len(fws)} frameworks, every k-th x-sheet of each grid and the "
-              f"first insertions of each Widom run ({evals:.3g} pair-evals, FP64 oracle, "
-              f"{threads} threads; errors over {n_grid_cmp} grid points with E < 1e4 K and "
-              f"{n_kh_cmp} Widom runs with a nonzero sum)")
+    sample = (f"{len(idx)} of {len(fws)} frameworks (every {every}-th), every k-th x-sheet of "
+              f"each grid and the first insertions of each Widom run ({evals:.3g} pair-evals, "
+              f"FP64 oracle, {threads} threads; rate extrapolated to the mix by pair-evals; "
+              f"errors over {n_grid_cmp} grid points with E < 1e4 K and {n_kh_cmp} Widom runs "
+              f"with a nonzero sum)")
     return dict(value=evals / secs, unit="pair-evals/s", cores=threads, kind="port",
                 sample=sample, seconds=round(secs, 2)), grid_err, kh_err, e_err
+
+
+def config1_full(api, synth, ctx, threads):
+    """The plain-method config (400-atom cubic cell, 50^3 grid, 1e5 insertions)
+    in full: the FP64 oracle on `threads` host threads and on one, and the
+    GPU's plain path on the same work, with the GPU's errors against it."""
+    from oracle import oracle
+    f = synth.config1()
+    a = f.particles()
+    n = f.grid_n()
+    img = (2 * f.rep[0] + 1) * (2 * f.rep[1] + 1) * (2 * f.rep[2] + 1)
+    pe = (int(np.prod(n)) + f.n_insertions) * f.n_atoms * img
+    out = {"workload": "config1: 400 atoms, 25 A cubic, 50^3 grid + 1e5 insertions at 298 K, "
+                       "FP64 oracle in full", "pair_evals": pe}
+    for th in sorted({threads, 1}, reverse=True):
+        t0 = time.perf_counter()
+        ref = oracle.lj_grid(f.t, a, f.rep, f.rc2, n, nthreads=th)
+        rb, reb = oracle.widom(f.t, a, f.rep, f.rc2, f.temperature, f.seed, 0, f.n_insertions,
+                               nthreads=th)
+        dt = time.perf_counter() - t0
+        out[f"cpu_seconds_{th}_threads"] = dt
+        out[f"cpu_pair_evals_per_s_{th}_threads"] = pe / dt
+    fw = api.Framework(ctx, a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2)
+    api.compute_grid(ctx, fw, n, api.ER_FP32)  # untimed first call
+    t0 = time.perf_counter()
+    g = api.Grid.compute(ctx, fw, n, api.ER_FP32)
+    sb, seb, _ = api.widom(ctx, fw, f.n_insertions, f.temperature, f.seed)
+    gpu_s = time.perf_counter() - t0
+    g32 = g.download().astype(np.float64)
+    g.close()
+    fw.close()
+    m = ref < 1e4
+    out.update(gpu_plain_seconds=gpu_s, gpu_pair_evals_per_s=pe / gpu_s,
+               gpu_over_cpu=out[f"cpu_seconds_{threads}_threads"] / gpu_s,
+               grid_max_abs_err_K=float(np.max(np.abs(g32[m] - ref[m]))) if m.any() else None,
+               kh_rel_err=abs(sb - rb) / rb if rb > 0 else None)
+    return out
 
 
 def percentile(v, q):
     return float(np.percentile(np.asarray(v), q)) if len(v) else None
 
 
+def serve_rate(api, synth, ctx, parts, rate, seconds, streams, items_per_launch):
+    """One Poisson stream at `rate` requests/s for about `seconds` into a fresh
+    server on ctx: grid / Henry requests on frameworks drawn (seeded) from the
+    pool; latency = completion - submission on the server's clock; grids stay
+    resident in HBM and are released unread."""
+    import threading
+    n_req = max(50, int(rate * seconds))
+    arrivals, kinds, temps = synth.config4_stream(rate, n_req)
+    pick = (synth.Rng(synth.framework_seed(4, 1)).u64(n_req) %
+            np.uint64(len(parts))).astype(np.int64)
+    srv = api.Server(ctx, streams=streams, items_per_launch=items_per_launch)
+    tickets, codes = [], []
+
+    def submit_all():
+        t0 = time.perf_counter()
+        for i in range(n_req):
+            dt = t0 + arrivals[i] - time.perf_counter()
+            if dt > 0:
+                time.sleep(dt)
+            f, a = parts[pick[i]]
+            if kinds[i] == 0:
+                code, tk = srv.submit(a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2,
+                                      grid_n=f.grid_n(), precision=api.ER_FP32)
+            else:
+                code, tk = srv.submit(a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2,
+                                      n_insertions=f.n_insertions,
+                                      temperature=float(temps[i]), seed=f.seed + i)
+            codes.append(code)
+            tickets.append(tk)
+
+    th = threading.Thread(target=submit_all)
+    th.start()
+    th.join()
+    lat, t_first, t_last, done = [], None, None, 0
+    for code, tk in zip(codes, tickets):
+        if code != 0:
+            continue
+        st = srv.poll(tk, wait_ms=-1)
+        if st.state == api.RQ_DONE:
+            done += 1
+            lat.append((st.t_done - st.t_submit) * 1e3)
+            t_first = st.t_submit if t_first is None else min(t_first, st.t_submit)
+            t_last = st.t_done if t_last is None else max(t_last, st.t_done)
+        srv.release(tk)
+    stats = srv.stats()
+    cache = srv.cache_stats()
+    srv.close()
+    span = (t_last - t_first) if done else float("nan")
+    return dict(rate=rate, requests=n_req, completed=done,
+                rejected=stats["rejected"], rejection_rate=stats["rejected"] / n_req,
+                p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99),
+                sustained_req_per_s=done / span if done else 0.0,
+                sustained_fws_per_s=done / span if done else 0.0,
+                launches=stats["launches"], framework_uploads=cache["misses"],
+                framework_cache_hits=cache["hits"],
+                resident_frameworks=cache["resident_frameworks"])
+
+
 def bench_serving(args, api, synth):
     """Poisson arrivals of grid / Henry requests into the continuous-batching
-    server (HBM cap 200 GB); latency = completion - submission on the
-    server's clock; grids stay resident in HBM and are released unread."""
-    import threading
+    server (HBM cap 200 GB)."""
     pool = [synth.config3_framework(i, config=4) for i in range(args.frameworks)]
     parts = [(f, f.particles()) for f in pool]
     ctx = api.Context(0, hbm_cap=200 << 30)
     rates = [float(r) for r in args.rates.split(",")]
     rows = []
     for rate in rates:
-        n_req = max(50, int(rate * args.serve_seconds))
-        arrivals, kinds, temps = synth.config4_stream(rate, n_req)
-        srv = api.Server(ctx, streams=args.streams, items_per_launch=args.items_per_launch)
-        tickets = []
-        codes = []
-
-        def submit_all():
-            t0 = time.perf_counter()
-            for i in range(n_req):
-                dt = t0 + arrivals[i] - time.perf_counter()
-                if dt > 0:
-                    time.sleep(dt)
-                f, a = parts[i % len(parts)]
-                if kinds[i] == 0:
-                    code, tk = srv.submit(a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2,
-                                          grid_n=f.grid_n(), precision=api.ER_FP32)
-                else:
-                    code, tk = srv.submit(a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2,
-                                          n_insertions=f.n_insertions,
-                                          temperature=float(temps[i]), seed=f.seed + i)
-                codes.append(code)
-                tickets.append(tk)
-
-        th = threading.Thread(target=submit_all)
-        th.start()
-        th.join()
-        lat, t_first, t_last, done = [], None, None, 0
-        for code, tk in zip(codes, tickets):
-            if code != 0:
-                continue
-            st = srv.poll(tk, wait_ms=-1)
-            if st.state == api.RQ_DONE:
-                done += 1
-                lat.append((st.t_done - st.t_submit) * 1e3)
-                t_first = st.t_submit if t_first is None else min(t_first, st.t_submit)
-                t_last = st.t_done if t_last is None else max(t_last, st.t_done)
-            srv.release(tk)
-        stats = srv.stats()
-        srv.close()
-        span = (t_last - t_first) if done else float("nan")
-        row = dict(rate=rate, requests=n_req, completed=done,
-                   rejected=stats["rejected"], rejection_rate=stats["rejected"] / n_req,
-                   p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99),
-                   sustained_fws_per_s=done / span if done else 0.0,
-                   launches=stats["launches"])
+        row = serve_rate(api, synth, ctx, parts, rate, args.serve_seconds, args.streams,
+                         args.items_per_launch)
         log(f"serving rate {rate:g}/s: {json.dumps(row)}")
         rows.append(row)
     ctx.close()
@@ -203,7 +289,8 @@ def bench_serving(args, api, synth):
         "dtype": "fp32 pair LJ beyond 1.5 sigma, fp64 r^2 / close pairs / accumulation",
         "data": "synthetic (seeded SplitMix64 frameworks, Poisson arrivals)",
         "config": {"workload": "config4: Poisson arrivals of grid / Henry requests, frameworks "
-                               "from the batching mix, T ~ U[250,350] K, HBM cap 200 GB",
+                               "drawn from a pool of the batching mix, T ~ U[250,350] K, "
+                               "HBM cap 200 GB",
                    "frameworks_in_pool": len(pool), "streams": args.streams,
                    "items_per_launch": args.items_per_launch,
                    "seconds_per_rate": args.serve_seconds, "parallelism": "single GPU"},
@@ -214,6 +301,7 @@ def bench_serving(args, api, synth):
 def bench_pockets(args, api, synth):
     """Config 5: FP32 grids at 0.2 A, pocket blocking at 15 T, then 1e7
     grid-interpolated insertions per framework (blocked voxels are 1e13 K)."""
+    from oracle import oracle
     T = 298.0
     fws = synth.config5(args.frameworks)
     ctx = api.Context(0)
@@ -225,6 +313,9 @@ def bench_pockets(args, api, synth):
         grids.append(api.Grid.compute(ctx, fw, f.grid_n(), api.ER_FP32))
         fwh.append(fw)
     t_grid = time.perf_counter() - t0
+    # unblocked copies of every 8th grid for the physics check below
+    check = list(range(0, len(fws), 8))
+    unblocked = {i: api.Grid.upload(ctx, grids[i].download()) for i in check}
     voxels = sum(int(np.prod(f.grid_n())) for f in fws)
     log(f"{len(fws)} grids, {voxels:.3g} voxels in {t_grid:.2f} s")
     t0 = time.perf_counter()
@@ -246,6 +337,38 @@ def bench_pockets(args, api, synth):
     t0 = time.perf_counter()
     api.grid_energies(ctx, grids[0], pos)
     t_en = time.perf_counter() - t0
+    # Physics check of the workload: grid-interpolated K_H against the direct
+    # FP64 sum (the reference's method on the GPU) at the direct method's
+    # first 1e6 insertion points, on every 8th framework; and the trilinear
+    # K_H of the blocked grid against the oracle at the same 1e7 points.
+    phys = []
+    m = 1_000_000
+    for i in check:
+        f = fws[i]
+        a = f.particles()
+        pts = oracle.widom_points(f.seed, 0, 0, m)
+        e_un = api.grid_energies(ctx, unblocked[i], pts)
+        e_bl = api.grid_energies(ctx, grids[i], pts)
+        _, _, _, de = api.direct_widom(ctx, a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2, m, T,
+                                       f.seed, energies=True)
+        k_dir = float(np.exp(-de / T).sum())
+        k_un = float(np.exp(-e_un / T).sum())
+        open_ = e_bl == e_un
+        k_dir_open = float(np.exp(-de[open_] / T).sum())
+        k_bl = float(np.exp(-e_bl / T).sum())
+        rb, _ = oracle.widom_grid(grids[i].download().astype(np.float64), T, f.seed, 0,
+                                  f.n_insertions)
+        kh_err = abs(kh[i] * f.n_insertions - rb) / rb if rb > 0 else \
+            (0.0 if kh[i] == 0 else float("inf"))
+        phys.append(dict(framework=f.name, pockets=int(pockets[i]),
+                         trilinear_vs_direct_rel=abs(k_un - k_dir) / k_dir if k_dir > 0 else None,
+                         blocked_vs_direct_outside_pockets_rel=(abs(k_bl - k_dir_open) / k_dir
+                                                                if k_dir > 0 else None),
+                         kh_rel_err_vs_oracle=kh_err))
+        unblocked[i].close()
+    devs = [p["trilinear_vs_direct_rel"] for p in phys if p["trilinear_vs_direct_rel"] is not None]
+    log(f"physics check on {len(phys)} frameworks: trilinear vs direct K_H median "
+        f"{np.median(devs):.3g}, max {max(devs):.3g}")
     for g in grids:
         g.close()
     for fw in fwh:
@@ -272,6 +395,14 @@ def bench_pockets(args, api, synth):
                      "kernel": "pocket labelling of all grids (one er_pockets_label_batch "
                                "call, 8 B/voxel)"},
         "trilinear_GBps_at_32B": tri_gbs,
+        "physics_check": {
+            "what": "K_H sums by trilinear interpolation of the 0.2 A FP32 grid vs the direct "
+                    "FP64 sum (reference method on the GPU), same 1e6 insertion points, every "
+                    "8th framework; blocked grid vs the direct sum outside blocked pockets",
+            "trilinear_vs_direct_rel_median": float(np.median(devs)),
+            "trilinear_vs_direct_rel_max": float(max(devs)),
+            "kh_rel_err_vs_oracle_max": max(p["kh_rel_err_vs_oracle"] for p in phys),
+            "frameworks": phys},
         "grid_energy_calls_per_s_host_buffers": len(pos) / t_en,
     }
 
@@ -318,6 +449,22 @@ def measured_traffic(tuning, n_frameworks):
         return None
 
 
+def batch_kernel_ms(api, ctx, tasks, runs):
+    """Mean HIP-event time of the tiled-kernel launches of a batch of tasks
+    (one untimed run first)."""
+    b = api.Batch(ctx, tasks, use_graph=False)
+    b.set_timing(True)
+    b.run()
+    b.wait()
+    ms = []
+    for _ in range(runs):
+        b.run()
+        b.wait()
+        ms.append(b.kernel_ms()[0])
+    b.close()
+    return float(np.mean(ms))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -327,6 +474,8 @@ def main():
     ap.add_argument("--frameworks", type=int, default=256)
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-fp64", action="store_true", help="skip the FP64-grid leg")
+    ap.add_argument("--no-serve", action="store_true", help="skip the serving leg")
     ap.add_argument("--grid-precision", default="fp32", choices=["fp32", "fp64"],
                     help="grids of the batch: fp32 (mixed precision) or fp64 (every pair "
                          "FP64, the drop-in writegrid's default)")
@@ -343,6 +492,9 @@ def main():
     ap.add_argument("--widom-per-bin", type=int, default=0)
     ap.add_argument("--rates", default="50,100,200,500", help="config4 arrival rates (req/s)")
     ap.add_argument("--serve-seconds", type=float, default=4.0)
+    ap.add_argument("--serve-rate", type=float, default=500.0,
+                    help="arrival rate of the serving leg of the default run (req/s)")
+    ap.add_argument("--serve-leg-seconds", type=float, default=2.0)
     ap.add_argument("--streams", type=int, default=3)
     ap.add_argument("--items-per-launch", type=int, default=32768)
     ap.add_argument("--tuning-file", default=DEFAULT_TUNING_FILE,
@@ -399,7 +551,6 @@ def main():
     gprec = api.ER_FP64 if args.grid_precision == "fp64" else api.ER_FP32
     tasks = [dict(fw=u, grid_n=f.grid_n(), precision=gprec, n_insertions=f.n_insertions,
                   temperature=f.temperature, seed=f.seed) for f, u in zip(fws, uploaded)]
-    tuned = None
     if args.autotune:
         from emptyroom_amd import autotune
         k = args.autotune
@@ -457,6 +608,38 @@ def main():
         gb.close()
         log(f"graph step {graph_ms:.1f} ms")
 
+    # The roofline kernel: the same tiled kernel on the step's grids alone.
+    # Every in-cutoff pair of a grid is evaluated (no early stop), so 16 FLOP
+    # per in-cutoff pair is work the kernel must do.  The Widom share of the
+    # step is timed on its own too (its insertions may stop once their energy
+    # is certain to reach 800 T, so it gets an insertion rate, not FLOP/s).
+    grid_tasks = [dict(t, n_insertions=0) for t in tasks]
+    widom_tasks = [dict(t, grid_n=(0, 0, 0)) for t in tasks]
+    grid_ms = batch_kernel_ms(api, ctx, grid_tasks, max(1, args.steps))
+    widom_ms = batch_kernel_ms(api, ctx, widom_tasks, max(1, args.steps))
+    log(f"grids alone {grid_ms:.1f} ms, Widom alone {widom_ms:.1f} ms (kernel)")
+
+    fp64 = None
+    if not args.no_fp64 and gprec == api.ER_FP32:
+        # the drop-in writegrid's default: FP64 grids (every pair FP64) with the
+        # same Widom runs, one batch per step; and the FP64 grids alone
+        t64 = [dict(t, precision=api.ER_FP64) for t in tasks]
+        b64 = api.Batch(ctx, t64, use_graph=False)
+        b64.run()
+        b64.wait()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            b64.run()
+            b64.wait()
+        t_fp64 = (time.perf_counter() - t0) / args.steps
+        # FP64 grids against the direct method are in the direct-method block
+        b64.close()
+        g64_ms = batch_kernel_ms(api, ctx, [dict(t, n_insertions=0) for t in t64],
+                                 max(1, args.steps))
+        fp64 = {"ms_per_step": t_fp64 * 1e3, "frameworks_per_s": len(fws) / t_fp64,
+                "grid_kernel_ms": g64_ms}
+        log(f"FP64-grid step {t_fp64 * 1e3:.1f} ms, FP64 grids alone {g64_ms:.1f} ms (kernel)")
+
     plain_s = None
     if not args.no_plain:
         # the same work as a batch step: grids stay resident in HBM (no
@@ -490,19 +673,33 @@ def main():
         est_s = t_direct * (grid_pe + widom_pe) / sample_pe
         # the batch's own results against the direct method's FP64 ones, at
         # every grid point and over every insertion of the sampled frameworks
-        gerr, kerr, eerr, n_pts, n_kh = 0.0, 0.0, 0.0, 0, 0
+        gerr, grel, g64err, kerr, eerr = 0.0, 0.0, 0.0, 0.0, 0.0
+        n_pts = n_hi = n_kh = n_kh0 = 0
         for i, gd, (db, deb, _) in zip(sample, d_grids, d_sums):
             ref = gd.download()
             gd.close()
+            g = batch.grid(i).astype(np.float64)
             m = ref < 1e4
             if m.any():
-                g = batch.grid(i).astype(np.float64)
                 gerr = max(gerr, float(np.max(np.abs(g[m] - ref[m]))))
                 n_pts += int(m.sum())
+            hi = ~m & (ref < 3.0e38)
+            if hi.any():
+                grel = max(grel, float(np.max(np.abs(g[hi] - ref[hi]) / ref[hi])))
+                n_hi += int(hi.sum())
+            if fp64 is not None:
+                g64 = api.compute_grid(ctx, uploaded[i], fws[i].grid_n(), api.ER_FP64)
+                g64err = max(g64err, float(np.max(np.abs(g64 - ref) /
+                                                  np.maximum(1.0, np.abs(ref)))))
             bb, beb, _ = batch.widom(i)
             if db > 0:
                 kerr = max(kerr, abs(bb - db) / db)
                 eerr = max(eerr, abs(beb / bb - deb / db))
+                n_kh += 1
+            elif bb == 0.0:
+                n_kh0 += 1
+            else:
+                kerr = float("inf")
                 n_kh += 1
         direct = {"sample": f"every {k}-th framework ({len(sample)} of {len(fws)}), grid + "
                             "Widom each, scaled to the mix by pair-evals",
@@ -513,42 +710,69 @@ def main():
                   "batched_errors": {
                       "grid_max_abs_err_K": gerr if n_pts else None,
                       "grid_points_compared": n_pts,
+                      "grid_max_rel_err_above_1e4K": grel if n_hi else None,
+                      "grid_points_above_1e4K_compared": n_hi,
+                      "fp64_grid_max_rel_err": g64err if fp64 is not None else None,
                       "kh_rel_err_max": kerr if n_kh else None,
                       "e_avg_max_abs_err_K": eerr if n_kh else None,
                       "widom_runs_compared": n_kh,
+                      "widom_runs_both_exactly_zero": n_kh0,
                       "against": "FP64 direct method (bit-identical to the oracle), every grid "
-                                 "point with E < 1e4 K and every insertion of the sample"}}
+                                 "point and every insertion of the sample"}}
         log(f"direct method {sample_pe / t_direct:.3g} pair-evals/s on the sample "
             f"({t_direct:.1f} s); whole mix ~{est_s:.1f} s; batched vs direct: grid "
-            f"{gerr:.3g} K over {n_pts} points, K_H {kerr:.3g} over {n_kh} runs")
+            f"{gerr:.3g} K over {n_pts} points (rel {grel:.3g} over {n_hi} above 1e4 K), "
+            f"K_H {kerr:.3g} over {n_kh} runs ({n_kh0} both 0)")
 
     # in-cutoff pairs of one step (exact GPU counting pass, outside the timed
-    # region).  The Widom share includes the pairs of insertions the kernel
-    # stops early (energy certain to reach 800 T, exp(-E/T) exactly 0), so
-    # `achieved` is the rate of the workload's algorithmic work, not of the
-    # pairs the hardware evaluated (the counter profile gives those).
+    # region), grids and Widom runs separately
     incut_grid = incut_widom = 0
     for f, u in zip(fws, uploaded):
         incut_grid += api.count_incut_grid(ctx, u, f.grid_n())
         incut_widom += api.count_incut_widom(ctx, u, f.n_insertions, f.seed)
     incut = incut_grid + incut_widom
-    kern_s = float(np.mean(kernel_ms)) / 1e3
-    achieved = FLOP_PER_PAIR * incut / kern_s / 1e12
+    achieved = FLOP_PER_PAIR * incut_grid / (grid_ms / 1e3) / 1e12
     log(f"in-cutoff pairs {incut:.4g} (grids {incut_grid:.4g}, Widom {incut_widom:.4g}); "
-        f"kernel {achieved:.2f} TFLOP/s")
+        f"grid kernel {achieved:.2f} TFLOP/s")
+    if fp64 is not None:
+        a64 = FLOP_PER_PAIR * incut_grid / (fp64["grid_kernel_ms"] / 1e3) / 1e12
+        fp64["roofline"] = {"bound": "valu", "achieved": a64, "peak": PEAK_FP64_TFLOPS,
+                            "unit": "TFLOP/s", "frac": a64 / PEAK_FP64_TFLOPS,
+                            "kernel": f"k_tiles<{tuning['ppl']},FP64> on the FP32 step's grids "
+                                      "in FP64, alone",
+                            "kernel_ms": fp64["grid_kernel_ms"]}
 
-    cpu = cpu1 = None
+    cpu = cpu1 = cfg1 = None
     grid_err = kh_err = e_err = None
+    host = host_cpu()
     if not args.no_cpu:
-        threads = min(16, os.cpu_count() or 1)
+        threads = host["threads"]
         cpu, grid_err, kh_err, e_err = cpu_baseline(fws, uploaded, ctx, batch, args.cpu_seconds,
                                                     threads)
-        log(f"cpu baseline {cpu['value']:.3g} pair-evals/s on {threads} threads")
+        cpu.update(nproc=host["nproc"], usable_cpus=host["usable"],
+                   omp_num_threads=host["omp_num_threads"], cpu_model=host["model"])
+        log(f"cpu baseline {cpu['value']:.3g} pair-evals/s on {threads} threads "
+            f"({host['model']}, nproc {host['nproc']}, usable {host['usable']})")
         # the same kind of sample on one core (a third of the time budget)
         cpu1, _, _, _ = cpu_baseline(fws, uploaded, ctx, batch, args.cpu_seconds / 3.0, 1)
         log(f"cpu baseline {cpu1['value']:.3g} pair-evals/s on 1 thread")
+        cfg1 = config1_full(api, synth, ctx, threads)
+        log(f"config1 in full: {json.dumps(cfg1)}")
+
+    serve = None
+    if not args.no_serve and args.workload == "config3":
+        pool = [synth.config3_framework(i, config=4) for i in range(64)]
+        parts = [(f, f.particles()) for f in pool]
+        sctx = api.Context(0, hbm_cap=200 << 30)
+        serve = serve_rate(api, synth, sctx, parts, args.serve_rate, args.serve_leg_seconds,
+                           args.streams, args.items_per_launch)
+        sctx.close()
+        serve["workload"] = ("config4: Poisson arrivals, frameworks drawn from a pool of 64 of "
+                             "the batching distribution, T ~ U[250,350] K, HBM cap 200 GB")
+        log(f"serving {json.dumps(serve)}")
 
     value = (grid_pe + widom_pe) / t_batched
+    ratio = (plain_s / t_batched) if plain_s else None
     out = {
         "metric": "LJ pair-evals/s + Widom inserts/s, batched vs plain, at K_H rel err <=1e-4",
         "value": value,
@@ -559,7 +783,9 @@ def main():
         "ms_per_step": t_batched * 1e3,
         "higher_is_better": True,
         "scaling": "weak",
-        "vs_baseline": None,
+        "vs_baseline": ratio / TARGET_BATCHED_OVER_PLAIN if ratio else None,
+        "vs_baseline_basis": f"batched/plain frameworks/s over the target "
+                             f"{TARGET_BATCHED_OVER_PLAIN:g}x (no published number exists)",
         "dtype": ("fp32 pair LJ beyond 1.5 sigma, fp64 r^2 / close pairs / accumulation"
                   if gprec == api.ER_FP32 else
                   "fp64 grids (every pair), Widom fp32 pair LJ beyond 1.5 sigma"),
@@ -572,30 +798,44 @@ def main():
         "grid_pair_evals_per_s": grid_pe / t_batched,
         "frameworks_per_s_batched": len(fws) / t_batched,
         "frameworks_per_s_plain": (len(fws) / plain_s) if plain_s else None,
-        "batched_over_plain": (plain_s / t_batched) if plain_s else None,
+        "batched_over_plain": ratio,
+        "batched_over_plain_target": TARGET_BATCHED_OVER_PLAIN,
+        "batched_over_plain_target_met": (ratio >= TARGET_BATCHED_OVER_PLAIN) if ratio else None,
         "direct_method": direct,
         "graph_ms_per_step": graph_ms,
+        "kernel_ms_per_step": float(np.mean(kernel_ms)),
         "incut_pairs_per_step": incut,
         "incut_pairs_grid": incut_grid,
         "incut_pairs_widom": incut_widom,
         "kh_rel_err_max": kh_err,
         "grid_max_abs_err_K": grid_err,
-        "roofline": {"bound": "mfma", "engine": "VALU FP32 (no MFMA: pairwise LJ)",
+        "roofline": {"bound": "valu", "engine": "VALU (no MFMA: pairwise LJ)",
                      "achieved": achieved, "peak": PEAK_FP32_TFLOPS, "unit": "TFLOP/s",
                      "frac": achieved / PEAK_FP32_TFLOPS,
+                     "attainable_peak": PEAK_FP32_UNPACKED_TFLOPS,
+                     "frac_attainable": achieved / PEAK_FP32_UNPACKED_TFLOPS,
                      "traffic": measured_traffic(tuning, len(fws))
                      if args.workload == "config3" and gprec == api.ER_FP32 else None,
-                     "kernel": f"k_tiles<{tuning['ppl']},MIXED>" if gprec == api.ER_FP32 else
-                               f"k_tiles<{tuning['ppl']},FP64> (grids) + "
-                               f"k_tiles<{tuning['ppl']},MIXED> (Widom)",
-                     "kernel_ms": kern_s * 1e3,
+                     "kernel": f"k_tiles<{tuning['ppl']},MIXED> on the step's {len(fws)} grids "
+                               "alone (one launch)" if gprec == api.ER_FP32 else
+                               f"k_tiles<{tuning['ppl']},FP64> on the step's grids alone",
+                     "kernel_ms": grid_ms,
                      "flop_per_pair": FLOP_PER_PAIR,
-                     "note": "16 FLOP x every in-cutoff pair of the step, including the Widom "
-                             "pairs of insertions stopped once their energy is certain to "
-                             "reach 800 T (exp(-E/T) exactly 0)"},
+                     "incut_pairs": incut_grid,
+                     "note": "16 FLOP x every in-cutoff pair of the step's grids (all of them "
+                             "evaluated) / the grid-only launch; peak 157.3 is the packed-FP32 "
+                             "rate, which this build does not use (attainable_peak 78.6)"},
+        "widom_alone": {"kernel_ms": widom_ms, "insertions_per_s": n_ins / (widom_ms / 1e3),
+                        "incut_pairs": incut_widom,
+                        "note": "insertions stop once their energy is certain to reach 800 T "
+                                "(exp(-E/T) exactly 0), so no FLOP rate is claimed"},
+        "fp64_grid_ms_per_step": fp64["ms_per_step"] if fp64 else None,
+        "fp64_grids": fp64,
+        "serving": serve,
         "tuning": tuning,
         "cpu_baseline": cpu,
         "cpu_baseline_1core": cpu1,
+        "config1_full": cfg1,
     }
     print(json.dumps(out))
     batch.close()

@@ -87,7 +87,8 @@ class Task(ctypes.Structure):
 
 class ServerConfig(ctypes.Structure):
     _fields_ = [("streams", ctypes.c_int), ("items_per_launch", ctypes.c_int64),
-                ("max_queue", ctypes.c_int)]
+                ("max_queue", ctypes.c_int), ("cache_bytes", ctypes.c_int64),
+                ("fail_launch", ctypes.c_int)]
 
 
 class Request(ctypes.Structure):
@@ -128,12 +129,13 @@ EXPORTS = [
     "er_context_get_tuning", "er_context_set_tuning", "er_context_memory",
     "er_framework_upload", "er_framework_release", "er_grid_compute", "er_grid_download",
     "er_grid_info", "er_grid_release", "er_widom", "er_count_incut_grid",
-    "er_count_incut_widom", "er_widom_point", "er_henry_result", "er_batch_create", "er_batch_run",
+    "er_count_incut_grid_map", "er_count_incut_widom", "er_widom_point", "er_henry_result", "er_batch_create", "er_batch_run",
     "er_batch_wait", "er_batch_result", "er_batch_release", "er_batch_set_timing",
     "er_batch_kernel_ms", "er_grid_upload", "er_pockets_label", "er_pockets_label_batch",
     "er_widom_grid",
     "er_server_create", "er_server_submit", "er_server_poll", "er_server_fetch_grid",
-    "er_server_release", "er_server_stats", "er_server_destroy", "er_host_setup",
+    "er_server_release", "er_server_stats", "er_server_cache_stats", "er_server_destroy",
+    "er_host_setup",
     "er_host_run_info", "er_host_run_particles", "er_host_run_label",
     "er_host_run_write_settings", "er_host_run_release", "er_write_grid_file",
     "er_replication_factors", "er_write_uc_file", "er_accessibility_grid",
@@ -169,6 +171,8 @@ _SIGS = {
                                 ctypes.POINTER(WidomResult), _P]),
     "er_count_incut_grid": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int),
                                            ctypes.POINTER(ctypes.c_uint64)]),
+    "er_count_incut_grid_map": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int),
+                                               ctypes.POINTER(ctypes.c_uint64), _P]),
     "er_count_incut_widom": (ctypes.c_int, [_P, _P, ctypes.c_uint64, ctypes.c_uint64,
                                             ctypes.POINTER(ctypes.c_uint64)]),
     "er_widom_point": (None, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
@@ -200,6 +204,7 @@ _SIGS = {
     "er_server_fetch_grid": (ctypes.c_int, [_P, ctypes.c_uint64, _P]),
     "er_server_release": (ctypes.c_int, [_P, ctypes.c_uint64]),
     "er_server_stats": (ctypes.c_int, [_P] + [ctypes.POINTER(ctypes.c_int64)] * 4),
+    "er_server_cache_stats": (ctypes.c_int, [_P] + [ctypes.POINTER(ctypes.c_int64)] * 5),
     "er_server_destroy": (ctypes.c_int, [_P]),
     "er_host_setup": (ctypes.c_int, [ctypes.c_char_p, ctypes.c_char_p, ctypes.c_char_p,
                                      ctypes.c_int, ctypes.POINTER(_P)]),
@@ -375,6 +380,15 @@ def count_incut_grid(ctx: Context, fw: Framework, n: Sequence[int]) -> int:
     v = ctypes.c_uint64()
     _check(lib().er_count_incut_grid(ctx.h, fw.h, _iarr(n), ctypes.byref(v)))
     return v.value
+
+
+def count_incut_grid_map(ctx: Context, fw: Framework, n: Sequence[int]):
+    """(total, per-point uint32 counts shaped like the grid)."""
+    v = ctypes.c_uint64()
+    out = np.empty(tuple(int(x) for x in n), dtype=np.uint32)
+    _check(lib().er_count_incut_grid_map(ctx.h, fw.h, _iarr(n), ctypes.byref(v),
+                                         out.ctypes.data_as(ctypes.c_void_p)))
+    return v.value, out
 
 
 def count_incut_widom(ctx: Context, fw: Framework, n: int, seed: int) -> int:
@@ -598,8 +612,8 @@ class Server:
     """Continuous-batching server on one context (er_server_*)."""
 
     def __init__(self, ctx: Context, streams: int = 3, items_per_launch: int = 32768,
-                 max_queue: int = 4096):
-        cfg = ServerConfig(streams, items_per_launch, max_queue)
+                 max_queue: int = 4096, cache_bytes: int = -1, fail_launch: int = 0):
+        cfg = ServerConfig(streams, items_per_launch, max_queue, cache_bytes, fail_launch)
         h = _P()
         _check(lib().er_server_create(ctx.h, ctypes.byref(cfg), ctypes.byref(h)))
         self.h = h
@@ -649,6 +663,12 @@ class Server:
         v = [ctypes.c_int64() for _ in range(4)]
         _check(lib().er_server_stats(self.h, *[ctypes.byref(x) for x in v]))
         return dict(zip(("submitted", "rejected", "completed", "launches"), [x.value for x in v]))
+
+    def cache_stats(self):
+        v = [ctypes.c_int64() for _ in range(5)]
+        _check(lib().er_server_cache_stats(self.h, *[ctypes.byref(x) for x in v]))
+        return dict(zip(("hits", "misses", "evictions", "resident_frameworks",
+                         "resident_bytes"), [x.value for x in v]))
 
     def close(self) -> None:
         if self.h:

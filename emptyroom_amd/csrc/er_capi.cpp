@@ -250,6 +250,18 @@ int er_count_incut_grid(er_context *ctx, er_framework *fw, const int n[3], uint6
     });
 }
 
+int er_count_incut_grid_map(er_context *ctx, er_framework *fw, const int n[3], uint64_t *total,
+                            uint32_t *counts_out) {
+    if (!ctx || !fw || !n || !counts_out)
+        return arg_error("er_count_incut_grid_map: null argument");
+    return guarded([&]() -> int {
+        unsigned long long v = 0;
+        int st = er::count_incut_grid(C(ctx), F(fw), n, &v, counts_out);
+        if (total) *total = v;
+        return st;
+    });
+}
+
 int er_count_incut_widom(er_context *ctx, er_framework *fw, uint64_t n_insertions, uint64_t seed,
                          uint64_t *out) {
     if (!ctx || !fw || !out) return arg_error("er_count_incut_widom: null argument");
@@ -402,9 +414,13 @@ int er_server_create(er_context *ctx, const er_server_config *config, er_server 
             cfg.streams = config->streams;
             cfg.items_per_launch = config->items_per_launch;
             cfg.max_queue = config->max_queue;
+            cfg.cache_bytes = config->cache_bytes;
+            cfg.fail_launch = config->fail_launch;
         }
-        if (cfg.streams < 1 || cfg.streams > 8 || cfg.items_per_launch < 0 || cfg.max_queue < 1)
-            return arg_error("er_server_create: streams 1..8, items_per_launch >= 0, max_queue >= 1");
+        if (cfg.streams < 1 || cfg.streams > 8 || cfg.items_per_launch < 0 || cfg.max_queue < 1 ||
+            cfg.cache_bytes < -1 || cfg.fail_launch < 0)
+            return arg_error("er_server_create: streams 1..8, items_per_launch >= 0, "
+                             "max_queue >= 1, cache_bytes >= -1, fail_launch >= 0");
         std::unique_ptr<er::Server> srv(new er::Server(C(ctx), cfg));
         if (srv->status() != ST_OK) {
             er::set_error("er_server_create: no device memory for the launch slots");
@@ -484,6 +500,20 @@ int er_server_stats(er_server *server, int64_t *submitted, int64_t *rejected, in
     if (completed) *completed = c;
     if (launches) *launches = d;
     return ST_OK;
+}
+
+int er_server_cache_stats(er_server *server, int64_t *hits, int64_t *misses, int64_t *evictions,
+                          int64_t *resident_frameworks, int64_t *resident_bytes) {
+    if (!server) return arg_error("er_server_cache_stats: null server");
+    return guarded([&]() -> int {
+        const er::CacheStats c = reinterpret_cast<er::Server *>(server)->cache_stats();
+        if (hits) *hits = c.hits;
+        if (misses) *misses = c.misses;
+        if (evictions) *evictions = c.evictions;
+        if (resident_frameworks) *resident_frameworks = c.resident_frameworks;
+        if (resident_bytes) *resident_bytes = c.resident_bytes;
+        return ST_OK;
+    });
 }
 
 int er_server_destroy(er_server *server) {

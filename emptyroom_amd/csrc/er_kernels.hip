@@ -761,7 +761,26 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
         using yes = std::integral_constant<bool, true>;
         for (int k = fb; k < fe; ++k) fp64_pair(L[k], no(), no());
         for (int k = sb; k < se; ++k) fp64_pair(Ls[-k], no(), yes());
-        for (int k = nb; k < ne; ++k) fp64_pair(Ln[k], yes(), yes());
+        // Near records (an atom within R of the tile centre, where a grid
+        // point may sit on the atom): r^2 from the FP64 differences, as
+        // src/computegridsheet.h:71-75 forms it, so that a point on an atom
+        // gives exactly 0 and the reference's clamp (the expanded form would
+        // leave a rounding residue of either sign there).
+        for (int k = nb; k < ne; ++k) {
+            const double4 A = Ln[k];
+#pragma unroll
+            for (int q = 0; q < PPL; ++q) {
+                const double dx = px[q] - A.x, dy = py[q] - A.y, dz = pz[q] - A.z;
+                const double rr = fmax(dx * dx + dy * dy + dz * dz, min_r2);  // :77
+                double y = (double)__builtin_amdgcn_rcpf((float)rr);
+                y = fma(fma(-rr, y, 1.0), y, y);
+                y = fma(fma(-rr, y, 1.0), y, y);
+                double y3 = y * y * y;
+                y3 = rr < rc2 ? y3 : 0.0;  // src/computegridsheet.h:79
+                b3[q] += y3;
+                b6[q] = fma(y3, y3, b6[q]);
+            }
+        }
 #pragma unroll
         for (int q = 0; q < PPL; ++q) acc[q] = fma(c6, b6[q], fma(c3, b3[q], acc[q]));
     }

@@ -831,7 +831,8 @@ int widom_run(Context *ctx, FrameworkDev *fw, uint64_t n_ins, double temperature
     return ST_OK;
 }
 
-int count_incut_grid(Context *ctx, FrameworkDev *fw, const int n[3], unsigned long long *out) {
+int count_incut_grid(Context *ctx, FrameworkDev *fw, const int n[3], unsigned long long *out,
+                     uint32_t *per_point_host) {
     ST_TRY(check_grid_dims(n));
     ST_TRY(check_tuning(ctx->tuning));
     HIP_TRY(hipSetDevice(ctx->device));
@@ -850,6 +851,9 @@ int count_incut_grid(Context *ctx, FrameworkDev *fw, const int n[3], unsigned lo
     HIP_TRY(launch_fill_u64(d_sum, 0ull, 1, s));
     HIP_TRY(launch_sum_u32(d_cnt, p, d_sum, s));
     HIP_TRY(hipMemcpyAsync(out, d_sum, sizeof(*out), hipMemcpyDeviceToHost, s));
+    if (per_point_host)
+        HIP_TRY(hipMemcpyAsync(per_point_host, d_cnt, sizeof(uint32_t) * p,
+                               hipMemcpyDeviceToHost, s));
     HIP_TRY(hipStreamSynchronize(s));
     return ST_OK;
 }

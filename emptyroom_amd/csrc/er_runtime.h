@@ -156,7 +156,9 @@ int grid_compute(Context *ctx, FrameworkDev *fw, const int n[3], int precision, 
 int widom_run(Context *ctx, FrameworkDev *fw, uint64_t n_ins, double temperature, uint64_t seed,
               WidomResult *out, double *energies_host);
 // In-cutoff (point, atom, image) triples of a grid / a Widom run.
-int count_incut_grid(Context *ctx, FrameworkDev *fw, const int n[3], unsigned long long *out);
+// per_point_host (nullable) receives each point's count, grid layout.
+int count_incut_grid(Context *ctx, FrameworkDev *fw, const int n[3], unsigned long long *out,
+                     uint32_t *per_point_host = nullptr);
 int count_incut_widom(Context *ctx, FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
                       unsigned long long *out);
 // Hard-sphere accessibility grid (src/computegridsheet.h:101-165): 1 where no

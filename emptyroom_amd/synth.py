@@ -2,10 +2,13 @@
 
 The random stream is SplitMix64 (Steele, Lea, Flood, OOPSLA'14) written out
 here with numpy integers, so every framework is a pure function of its seed on
-any machine.  Cell geometry follows src/Framework.cc:113-142, mixing
-src/load_fast_particle_f_array.h:35-36, and the replication rule is the
-"once" rule of julia_codes/framework.jl:162-211 restated as: the smallest
-integer r_i with r_i * w_i > 2 r_c, w_i the perpendicular width of the cell.
+any machine.  Cell geometry follows src/Framework.cc:113-142 and mixing
+src/load_fast_particle_f_array.h:35-36.  The replication factors of the
+synthetic workloads come from their own rule: the smallest integer r_i with
+r_i * w_i > 2 r_c, w_i the perpendicular width of the cell (`replication`;
+the image half-range must reach r_c past every face of the cell).  This is
+not the doubling rule of julia_codes/framework.jl:162-211, which the drop-ins
+use through er_replication_factors; the two can differ by one image layer.
 """
 from __future__ import annotations
 
@@ -219,15 +222,18 @@ def config4_stream(rate: float, n_requests: int, seed: int = None):
     return arrivals, kinds, temps
 
 
+def config5_framework(index: int) -> SynthFramework:
+    """Framework `index` of the pocket-blocking workload: drawn from the batching
+    distribution, on a 0.2 A grid with 1e7 trilinear insertions."""
+    f = config3_framework(index, config=5)
+    f.spacing = 0.2
+    f.n_insertions = 10_000_000
+    return f
+
+
 def config5(count: int = 64) -> List[SynthFramework]:
     """Frameworks from the batching distribution on a 0.2 A grid with 1e7 trilinear insertions."""
-    out = []
-    for i in range(count):
-        f = config3_framework(i, config=5)
-        f.spacing = 0.2
-        f.n_insertions = 10_000_000
-        out.append(f)
-    return out
+    return [config5_framework(i) for i in range(count)]
 
 
 def small_framework(seed: int, n_atoms: int = 30, cell=(11.0, 12.0, 13.0, 80.0, 95.0, 105.0),

@@ -163,6 +163,11 @@ void er_widom_point(uint64_t seed, uint32_t stream, uint64_t i, double f[3]);
 /* Number of (point, atom, image) triples with r^2 < r_c^2: the work unit of
  * the roofline figures.  Uses the same tile lists as the energy kernels. */
 int er_count_incut_grid(er_context *ctx, er_framework *fw, const int n[3], uint64_t *out);
+/* The same count per grid point (uint32, grid layout k + j*N_z + i*N_y*N_z)
+ * and its total (nullable): the in-cutoff test of src/computegridsheet.h:79
+ * counted point by point. */
+int er_count_incut_grid_map(er_context *ctx, er_framework *fw, const int n[3], uint64_t *total,
+                            uint32_t *counts_out);
 int er_count_incut_widom(er_context *ctx, er_framework *fw, uint64_t n_insertions, uint64_t seed,
                          uint64_t *out);
 
@@ -233,17 +238,28 @@ int er_grid_energies(er_context *ctx, const er_grid *grid, const double *frac_xy
                      uint64_t n_positions, double *energies_out);
 
 /* ---- serving ------------------------------------------------------------------
- * No reference counterpart.  A server owns a scheduler thread on one context:
- * requests (a framework plus a grid and/or a Widom run) are admitted in
- * arrival order while the context's HBM cap allows, and every launch of the
- * tiled kernel takes pending tiles / bins from all admitted requests, so work
- * of requests that arrive while others run overlaps on the device.  Results
- * are bit-identical to the plain path.  While a server runs, use its context
- * only through the server. */
+ * No reference counterpart (the reference runs one framework per process and
+ * uploads it per run, src/writegrid.cu:186-189, src/henry.cu:133-135).  A
+ * server owns a scheduler thread on one context: requests (a framework plus a
+ * grid and/or a Widom run) are admitted in arrival order while the HBM budget
+ * allows (the context's cap, or the device memory free at server creation
+ * less a margin), and every launch of the tiled kernel takes pending tiles /
+ * bins from all admitted requests, so work of requests that arrive while
+ * others run overlaps on the device.  Frameworks stay resident in HBM: an
+ * upload is shared by every request with the same atoms, cell, replication
+ * and cutoff, and kept after its requests finish until admission needs the
+ * memory (least recently used first) or the idle ones exceed cache_bytes.
+ * Each launch computes FP64 grids or everything else, chosen by the oldest
+ * ready request's pending work.  Results are bit-identical to the plain
+ * path.  While a server runs, use its context only through the server. */
 typedef struct er_server_config {
     int streams;               /* launches in flight (1..8) */
     int64_t items_per_launch;  /* tiles + bins per launch, 0 = all pending */
     int max_queue;             /* waiting requests beyond which submit rejects */
+    int64_t cache_bytes;       /* idle frameworks kept resident: -1 up to the budget, 0 none */
+    int fail_launch;           /* failure-path testing: the n-th launch attempt is refused
+                                  on the host before anything is enqueued, and the requests
+                                  holding its items fail with ER_E_HIP (0 = never) */
 } er_server_config;
 
 typedef struct er_request {
@@ -280,6 +296,10 @@ int er_server_fetch_grid(er_server *server, uint64_t ticket, void *host_out);
 int er_server_release(er_server *server, uint64_t ticket);
 int er_server_stats(er_server *server, int64_t *submitted, int64_t *rejected, int64_t *completed,
                     int64_t *launches);
+/* Framework residency: uploads avoided (hits), uploads made (misses),
+ * frameworks evicted, and the frameworks / bytes resident now. */
+int er_server_cache_stats(er_server *server, int64_t *hits, int64_t *misses, int64_t *evictions,
+                          int64_t *resident_frameworks, int64_t *resident_bytes);
 int er_server_destroy(er_server *server);
 
 /* ---- host side of the drop-in binaries -------------------------------------

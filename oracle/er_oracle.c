@@ -58,6 +58,16 @@ void orc_widom_point(uint64_t seed, uint32_t stream, uint64_t i, double f[3]) {
     for (int k = 0; k < 3; k++) f[k] = ((double)w[k] + 1.0) * inv2_32;
 }
 
+static void set_threads(int nthreads);
+
+/* Points i0 .. i0 + n - 1 of one stream, f[3 n]. */
+void orc_widom_points(uint64_t seed, uint32_t stream, uint64_t i0, int64_t n, double *f,
+                      int nthreads) {
+    set_threads(nthreads);
+#pragma omp parallel for schedule(static)
+    for (int64_t p = 0; p < n; p++) orc_widom_point(seed, stream, i0 + (uint64_t)p, f + 3 * p);
+}
+
 /* ------------------------------------------------------------------ */
 /* Cell geometry and mixing                                            */
 /* ------------------------------------------------------------------ */

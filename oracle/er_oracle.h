@@ -31,6 +31,8 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
  * Replaces curand_uniform_double on MTGP32 (src/doHenry.h:53-55), whose
  * stream is closed and is not reproduced. */
 void orc_widom_point(uint64_t seed, uint32_t stream, uint64_t i, double f[3]);
+void orc_widom_points(uint64_t seed, uint32_t stream, uint64_t i0, int64_t n, double *f,
+                      int nthreads);
 
 /* Fractional->Cartesian matrix, its inverse and the cell volume, as
  * src/Framework.cc:113-142 (angles in degrees, converted as :35-37). */

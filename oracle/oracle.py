@@ -35,6 +35,8 @@ def lib():
         u32p = ctypes.POINTER(ctypes.c_uint32)
         L.orc_philox4x32_10.argtypes = [u32p, u32p, u32p]
         L.orc_widom_point.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64, _D]
+        L.orc_widom_points.argtypes = [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,
+                                       ctypes.c_int64, _D, ctypes.c_int]
         L.orc_cell_matrices.argtypes = [ctypes.c_double] * 6 + [_D, _D, _D]
         L.orc_lb_mix.argtypes = [ctypes.c_double] * 4 + [_D, _D]
         L.orc_image_atoms.argtypes = [_D, _D, ctypes.c_int, _I, _D]
@@ -93,6 +95,13 @@ def philox(ctr, key):
 def widom_point(seed: int, stream: int, i: int) -> np.ndarray:
     f = np.empty(3)
     lib().orc_widom_point(seed, stream, i, _d(f))
+    return f
+
+
+def widom_points(seed: int, stream: int, i0: int, n: int, nthreads=0) -> np.ndarray:
+    """Insertion points i0 .. i0 + n - 1 of stream (seed, stream), shape (n, 3)."""
+    f = np.empty((int(n), 3))
+    lib().orc_widom_points(seed, stream, i0, int(n), _d(f), nthreads)
     return f
 
 

@@ -1,0 +1,230 @@
+"""The HIP path on the workloads of the precision, batching and pocket-blocking
+configurations, against the oracle (or the reference's direct method on the
+GPU, which equals the oracle bit for bit -- tests/test_gpu_direct.py).
+
+Precision (one 3000-atom triclinic framework, 0.1 A grid of 6.5e6 points,
+1e7 insertions; src/computegridsheet.h:71-82, src/doHenry.h:58-107):
+- FP32 vs FP64 at every point: 1e-3 K where E < 1e4 K, 1e-6 relative above
+  (FLT_MAX where the FP64 energy exceeds it);
+- FP64 vs the oracle at 3000 sampled points (1e-9 relative);
+- per-point in-cutoff counts, summed over sampled x-sheets, equal the oracle's;
+- K_H from 1e7 insertions vs the direct method at the same points (1e-4
+  relative; both exactly 0 allowed) and every insertion energy.
+The drawn cell holds 0.48 atoms per A^3, so nearly every point and insertion
+is inside an atom: the tests report how many points fall below 1e4 K.
+
+Batching (16 frameworks of the 256-framework mix with all three spacings and
+guests, one batch): every grid at 400 sampled points and every K_H (20000
+insertions) against the oracle; each result equal to the plain path bit for
+bit.
+
+Pocket blocking (every 8th framework of the 64 at 0.2 A, full size,
+src/pocketblocking.cc:126-391, src/gcmc.cc:145-177): segment ids, pocket
+count and the blocked grid bit-exact vs the oracle on the same FP32 grid;
+K_H of 1e7 trilinear insertions vs the oracle at the same points.  The
+physics check of the workload -- grid-interpolated K_H against the direct
+FP64 sum -- at the first 1e6 insertion points of the direct method: the
+trilinear interpolation of a 0.2 A grid is not exact, so the bound is a
+stated physics bound (relative deviation of the unblocked trilinear K_H
+within PHYSICS_BOUND), and the blocked-grid K_H is compared with the direct
+sum over the points outside blocked pockets.
+"""
+import numpy as np
+import pytest
+
+from emptyroom_amd import api, synth
+from oracle import oracle
+
+pytestmark = pytest.mark.gpu
+
+FLT_MAX = float(np.finfo(np.float32).max)
+# Relative deviation of grid-interpolated (0.2 A, trilinear) K_H from the
+# direct FP64 sum at the same insertion points.
+PHYSICS_BOUND = 0.5
+
+
+def fp32_vs_fp64(g32, g64):
+    """(points below 1e4 K, max abs error there, max relative error above)."""
+    g32 = g32.astype(np.float64)
+    lo = g64 < 1e4
+    err_lo = float(np.max(np.abs(g32[lo] - g64[lo]))) if lo.any() else 0.0
+    hi = ~lo & (g64 < FLT_MAX)
+    err_hi = float(np.max(np.abs(g32[hi] - g64[hi]) / g64[hi])) if hi.any() else 0.0
+    over = g64 >= FLT_MAX
+    assert np.all(g32[over] == FLT_MAX)
+    return int(lo.sum()), err_lo, err_hi
+
+
+# ---------------------------------------------------------------------------
+# precision workload
+# ---------------------------------------------------------------------------
+
+@pytest.fixture(scope="module")
+def cfg2(ctx):
+    f = synth.config2()
+    a = f.particles()
+    fw = api.Framework(ctx, a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2)
+    yield f, a, fw
+    fw.close()
+
+
+def test_config2_grid_full_size(ctx, cfg2):
+    f, a, fw = cfg2
+    n = f.grid_n()
+    print(f"config2 grid {n} ({np.prod(n):.3g} points), {len(a)} atoms, rep {f.rep}", flush=True)
+    ctx.tuning = api.default_tuning()
+    g64 = api.compute_grid(ctx, fw, n, api.ER_FP64)
+    g32 = api.compute_grid(ctx, fw, n, api.ER_FP32)
+    nlo, err_lo, err_hi = fp32_vs_fp64(g32, g64)
+    print(f"config2: {nlo} points below 1e4 K (max err {err_lo:.3g} K), "
+          f"max rel err above {err_hi:.3g}", flush=True)
+    assert err_lo <= 1e-3 and err_hi < 1e-6
+    rng = np.random.default_rng(5)
+    idx = np.stack([rng.integers(0, n[d], 3000) for d in range(3)], 1)
+    ref = oracle.lj_points(f.t, a, f.rep, f.rc2, 1e-24, idx / (np.array(n) - 1.0))
+    got = g64[idx[:, 0], idx[:, 1], idx[:, 2]]
+    rel = np.abs(got - ref) / np.maximum(1.0, np.abs(ref))
+    assert rel.max() < 1e-9, rel.max()
+
+
+def test_config2_counts_on_sampled_sheets(ctx, cfg2):
+    f, a, fw = cfg2
+    n = f.grid_n()
+    total, cnt = api.count_incut_grid_map(ctx, fw, n)
+    assert total == int(cnt.sum(dtype=np.uint64))
+    assert total == api.count_incut_grid(ctx, fw, n)
+    for s in (0, n[0] // 2, n[0] - 1):
+        print(f"config2 counts sheet {s}", flush=True)
+        ref = oracle.count_incut_grid(f.t, a, f.rep, f.rc2, n, sheets=(s, s + 1, 1))
+        assert int(cnt[s].sum(dtype=np.uint64)) == ref
+
+
+def test_config2_widom_1e7_vs_direct(ctx, cfg2):
+    f, a, fw = cfg2
+    nins, T = f.n_insertions, f.temperature
+    print(f"config2 Widom {nins} insertions", flush=True)
+    ctx.tuning = api.default_tuning()
+    sb, seb, n, e = api.widom(ctx, fw, nins, T, f.seed, energies=True)
+    db, deb, _, de = api.direct_widom(ctx, a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2, nins,
+                                      T, f.seed, energies=True)
+    assert n == nins
+    lo = de < 1e4
+    if lo.any():
+        assert np.max(np.abs(e[lo] - de[lo])) <= 1e-3
+    fin = ~lo & np.isfinite(de)
+    assert np.max(np.abs(e[fin] - de[fin]) / de[fin]) < 1e-6
+    print(f"config2: {int(lo.sum())} insertions below 1e4 K; K_H sums {sb:.6g} vs {db:.6g}",
+          flush=True)
+    if db > 0:
+        assert abs(sb - db) / db <= 1e-4
+    else:
+        assert sb == 0.0
+    # the sums without the per-insertion energies (saturation stop on) are the same
+    sb2, seb2, _ = api.widom(ctx, fw, nins, T, f.seed)
+    assert sb2 == sb and seb2 == seb
+
+
+# ---------------------------------------------------------------------------
+# batching workload
+# ---------------------------------------------------------------------------
+
+def test_config3_batch_against_oracle(ctx):
+    mix = synth.config3(256)
+    pick = list(range(3, 256, 16))
+    fs = [mix[i] for i in pick]
+    assert {f.spacing for f in fs} == {0.1, 0.2, 0.5} and {f.guest for f in fs} == \
+        {"CH4", "Xe", "Kr"}
+    ctx.tuning = api.default_tuning()
+    fws = []
+    for f in fs:
+        a = f.particles()
+        fws.append(api.Framework(ctx, a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2))
+    nins = 20000
+    tasks = [dict(fw=fw, grid_n=f.grid_n(), precision=api.ER_FP32, n_insertions=nins,
+                  temperature=f.temperature, seed=f.seed) for f, fw in zip(fs, fws)]
+    b = api.Batch(ctx, tasks, use_graph=True)
+    b.run()
+    b.wait()
+    rng = np.random.default_rng(17)
+    for i, (f, fw) in enumerate(zip(fs, fws)):
+        a = f.particles()
+        n = f.grid_n()
+        print(f"batch {f.name}: {len(a)} atoms, grid {n}, spacing {f.spacing}, {f.guest}",
+              flush=True)
+        g = b.grid(i)
+        idx = np.stack([rng.integers(0, n[d], 400) for d in range(3)], 1)
+        ref = oracle.lj_points(f.t, a, f.rep, f.rc2, 1e-24, idx / (np.array(n) - 1.0))
+        got = g[idx[:, 0], idx[:, 1], idx[:, 2]].astype(np.float64)
+        lo = ref < 1e4
+        if lo.any():
+            assert np.max(np.abs(got[lo] - ref[lo])) <= 1e-3
+        hi = ~lo & (ref < FLT_MAX)
+        if hi.any():
+            assert np.max(np.abs(got[hi] - ref[hi]) / ref[hi]) < 1e-6
+        sb, seb, _ = b.widom(i)
+        rb, reb = oracle.widom(f.t, a, f.rep, f.rc2, f.temperature, f.seed, 0, nins)
+        if rb > 0:
+            assert abs(sb - rb) / rb <= 1e-4
+            assert abs(seb / sb - reb / rb) <= 1e-3
+        else:
+            assert sb == 0.0
+        # alone == in the batch
+        assert np.array_equal(g, api.compute_grid(ctx, fw, n, api.ER_FP32))
+        assert (sb, seb, nins) == api.widom(ctx, fw, nins, f.temperature, f.seed)
+    b.close()
+    for fw in fws:
+        fw.close()
+
+
+# ---------------------------------------------------------------------------
+# pocket-blocking workload
+# ---------------------------------------------------------------------------
+
+@pytest.mark.parametrize("idx", list(range(0, 64, 8)))
+def test_config5_framework(ctx, idx):
+    T = 298.0
+    f = synth.config5_framework(idx)
+    a = f.particles()
+    n = f.grid_n()
+    print(f"config5 {f.name}: {len(a)} atoms, grid {n}", flush=True)
+    ctx.tuning = api.default_tuning()
+    args = (a[:, :3], a[:, 3], a[:, 4], f.t, f.rep, f.rc2)
+    fw = api.Framework(ctx, *args)
+    grid = api.Grid.compute(ctx, fw, n, api.ER_FP32)
+    g32 = grid.download()
+    unblocked = api.Grid.upload(ctx, g32)
+    ref_n, ref_blocked, ref_seg = oracle.find_and_block_pockets(g32.astype(np.float64), T)
+    npk, seg = api.pockets_label(ctx, grid, T, block=True, segments=True)
+    blocked = grid.download()
+    assert npk == ref_n
+    assert np.array_equal(seg, ref_seg)
+    assert np.array_equal(blocked, ref_blocked.astype(np.float32))
+    nins = f.n_insertions
+    sb, seb, nn = api.widom_grid(ctx, grid, nins, T, f.seed)
+    rb, reb = oracle.widom_grid(blocked.astype(np.float64), T, f.seed, 0, nins)
+    assert nn == nins
+    if rb > 0:
+        assert abs(sb - rb) / rb < 1e-9
+    else:
+        assert sb == 0.0
+    # physics check: grid-interpolated vs direct FP64 at the direct method's points
+    m = 1_000_000
+    pts = oracle.widom_points(f.seed, 0, 0, m)
+    e_un = api.grid_energies(ctx, unblocked, pts)
+    e_bl = api.grid_energies(ctx, grid, pts)
+    db, _, _, de = api.direct_widom(ctx, *args, m, T, f.seed, energies=True)
+    k_dir = float(np.exp(-de / T).sum())
+    k_un = float(np.exp(-e_un / T).sum())
+    in_pocket = e_bl != e_un
+    k_dir_open = float(np.exp(-de[~in_pocket] / T).sum())
+    k_bl = float(np.exp(-e_bl / T).sum())
+    dev = abs(k_un - k_dir) / k_dir if k_dir > 0 else 0.0
+    dev_bl = abs(k_bl - k_dir_open) / k_dir if k_dir > 0 else 0.0
+    print(f"config5 {f.name}: {npk} pockets, K_H sums direct {k_dir:.6g} trilinear {k_un:.6g} "
+          f"(rel {dev:.3g}); outside pockets direct {k_dir_open:.6g} blocked-grid {k_bl:.6g} "
+          f"(dev / K_direct {dev_bl:.3g})", flush=True)
+    assert abs(db - k_dir) <= 1e-9 * max(k_dir, 1e-300)
+    assert dev <= PHYSICS_BOUND and dev_bl <= PHYSICS_BOUND
+    for h in (grid, unblocked):
+        h.close()
+    fw.close()
