@@ -360,10 +360,12 @@ def bench_pockets(args, api, synth):
                                   f.n_insertions)
         kh_err = abs(kh[i] * f.n_insertions - rb) / rb if rb > 0 else \
             (0.0 if kh[i] == 0 else float("inf"))
-        phys.append(dict(framework=f.name, pockets=int(pockets[i]),
-                         trilinear_vs_direct_rel=abs(k_un - k_dir) / k_dir if k_dir > 0 else None,
+        acc = k_dir > 1e-10 * m  # accessible space (mean Boltzmann factor above 1e-10)
+        phys.append(dict(framework=f.name, pockets=int(pockets[i]), accessible=bool(acc),
+                         mean_b_direct=k_dir / m, mean_b_trilinear=k_un / m,
+                         trilinear_vs_direct_rel=abs(k_un - k_dir) / k_dir if acc else None,
                          blocked_vs_direct_outside_pockets_rel=(abs(k_bl - k_dir_open) / k_dir
-                                                                if k_dir > 0 else None),
+                                                                if acc else None),
                          kh_rel_err_vs_oracle=kh_err))
         unblocked[i].close()
     devs = [p["trilinear_vs_direct_rel"] for p in phys if p["trilinear_vs_direct_rel"] is not None]

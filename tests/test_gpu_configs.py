@@ -26,8 +26,12 @@ physics check of the workload -- grid-interpolated K_H against the direct
 FP64 sum -- at the first 1e6 insertion points of the direct method: the
 trilinear interpolation of a 0.2 A grid is not exact, so the bound is a
 stated physics bound (relative deviation of the unblocked trilinear K_H
-within PHYSICS_BOUND), and the blocked-grid K_H is compared with the direct
-sum over the points outside blocked pockets.
+within PHYSICS_BOUND, for frameworks with accessible space), and the
+blocked-grid K_H is compared with the direct sum over the points outside
+blocked pockets.  Trilinear interpolation overestimates the energy in the
+wells (the potential is convex there), so it gives the smaller K_H: on these
+randomly packed frameworks, with wells of -1000 to -2500 K, the deviation
+is 10-30 %.
 """
 import numpy as np
 import pytest
@@ -41,6 +45,9 @@ FLT_MAX = float(np.finfo(np.float32).max)
 # Relative deviation of grid-interpolated (0.2 A, trilinear) K_H from the
 # direct FP64 sum at the same insertion points.
 PHYSICS_BOUND = 0.5
+# The check applies where the mean Boltzmann factor exceeds this (below it
+# the framework is blocked nearly everywhere and both sums are ~0).
+ACCESSIBLE = 1e-10
 
 
 def fp32_vs_fp64(g32, g64):
@@ -224,7 +231,8 @@ def test_config5_framework(ctx, idx):
           f"(rel {dev:.3g}); outside pockets direct {k_dir_open:.6g} blocked-grid {k_bl:.6g} "
           f"(dev / K_direct {dev_bl:.3g})", flush=True)
     assert abs(db - k_dir) <= 1e-9 * max(k_dir, 1e-300)
-    assert dev <= PHYSICS_BOUND and dev_bl <= PHYSICS_BOUND
+    if k_dir > ACCESSIBLE * m:  # a framework with accessible space
+        assert dev <= PHYSICS_BOUND and dev_bl <= PHYSICS_BOUND
     for h in (grid, unblocked):
         h.close()
     fw.close()
