@@ -51,6 +51,15 @@ def test_product_stream_equals_oracle_stream():
                               oracle.widom_point(seed, stream, i))
 
 
+def test_bulk_points_equal_single_points():
+    """orc_widom_points (used to replay the direct method's insertion points on
+    the host) equals orc_widom_point index by index, across a 2^32 boundary."""
+    for seed, stream, i0 in [(77, 0, 0), (2 ** 40 + 3, 1, 2 ** 32 - 50)]:
+        bulk = oracle.widom_points(seed, stream, i0, 200, nthreads=2)
+        one = np.array([oracle.widom_point(seed, stream, i0 + k) for k in range(200)])
+        assert np.array_equal(bulk, one)
+
+
 # --------------------------------------------------------------------------
 # pocket blocking
 # --------------------------------------------------------------------------
