@@ -129,7 +129,8 @@ EXPORTS = [
     "er_context_get_tuning", "er_context_set_tuning", "er_context_memory",
     "er_framework_upload", "er_framework_release", "er_grid_compute", "er_grid_download",
     "er_grid_info", "er_grid_release", "er_widom", "er_count_incut_grid",
-    "er_count_incut_grid_map", "er_count_incut_widom", "er_widom_point", "er_henry_result", "er_batch_create", "er_batch_run",
+    "er_count_incut_grid_map", "er_count_incut_widom", "er_widom_point",
+    "er_grid_compute_noalloc", "er_henry_result", "er_batch_create", "er_batch_run",
     "er_batch_wait", "er_batch_result", "er_batch_release", "er_batch_set_timing",
     "er_batch_kernel_ms", "er_grid_upload", "er_pockets_label", "er_pockets_label_batch",
     "er_widom_grid",
@@ -173,6 +174,8 @@ _SIGS = {
                                            ctypes.POINTER(ctypes.c_uint64)]),
     "er_count_incut_grid_map": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int),
                                                ctypes.POINTER(ctypes.c_uint64), _P]),
+    "er_grid_compute_noalloc": (ctypes.c_int, [_P, _P, ctypes.POINTER(ctypes.c_int),
+                                               ctypes.c_int, _P]),
     "er_count_incut_widom": (ctypes.c_int, [_P, _P, ctypes.c_uint64, ctypes.c_uint64,
                                             ctypes.POINTER(ctypes.c_uint64)]),
     "er_widom_point": (None, [ctypes.c_uint64, ctypes.c_uint32, ctypes.c_uint64,

@@ -250,6 +250,15 @@ int er_count_incut_grid(er_context *ctx, er_framework *fw, const int n[3], uint6
     });
 }
 
+int er_grid_compute_noalloc(er_context *ctx, er_framework *fw, const int n[3], int precision,
+                            void *device_out) {
+    if (!ctx || !fw || !n || !device_out)
+        return arg_error("er_grid_compute_noalloc: null argument");
+    return guarded([&]() -> int {
+        return er::grid_compute_noalloc(C(ctx), F(fw), n, precision, device_out);
+    });
+}
+
 int er_count_incut_grid_map(er_context *ctx, er_framework *fw, const int n[3], uint64_t *total,
                             uint32_t *counts_out) {
     if (!ctx || !fw || !n || !counts_out)

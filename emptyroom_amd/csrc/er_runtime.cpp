@@ -471,7 +471,7 @@ void choose_widom_bins(const FrameworkDev *fw, uint64_t n_ins, const Tuning &tu,
 // the cells per axis.  Regions of one cell skip the sub-list.  A cell's
 // candidate list, and so every result, is the same whatever region it is in.
 void choose_region(const FrameworkDev *fw, const double cell_frac[3], const int ncell[3],
-                   const Tuning &tu, long long min_items, int region[3]) {
+                   const Tuning &tu, long long min_items, int region[3], int slice_records) {
     double edge[3];
     for (int d = 0; d < 3; d++) {
         const double a = std::sqrt(fw->t[d] * fw->t[d] + fw->t[3 + d] * fw->t[3 + d] +
@@ -501,7 +501,7 @@ void choose_region(const FrameworkDev *fw, const double cell_frac[3], const int 
         const double expect = dens * (4.0 / 3.0) * pi * rr * rr * rr;
         long long items = 1;
         for (int d = 0; d < 3; d++) items *= (ncell[d] + region[d] - 1) / region[d];
-        if (expect <= 0.75 * kRegionRecords && items >= min_items) break;
+        if (expect <= 0.75 * slice_records && items >= min_items) break;
         int big = 0;
         for (int d = 1; d < 3; d++)
             if (region[d] * edge[d] > region[big] * edge[big]) big = d;
@@ -511,18 +511,18 @@ void choose_region(const FrameworkDev *fw, const double cell_frac[3], const int 
 }
 
 void choose_grid_region(const FrameworkDev *fw, const int n[3], const int tile[3],
-                        const Tuning &tu, int region[3]) {
+                        const Tuning &tu, int region[3], int slice_records) {
     double frac[3];
     int nt[3];
     for (int d = 0; d < 3; d++) {
         frac[d] = (double)tile[d] / (n[d] - 1);
         nt[d] = (n[d] + tile[d] - 1) / tile[d];
     }
-    choose_region(fw, frac, nt, tu, kMinGridItems, region);
+    choose_region(fw, frac, nt, tu, kMinGridItems, region, slice_records);
 }
 
 JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void *d_out,
-                     const Tuning &tu, int math) {
+                     const Tuning &tu, int math, int slice_records) {
     JobDev j;
     std::memset(&j, 0, sizeof(j));
     std::memcpy(j.t, fw->t, sizeof(j.t));
@@ -537,7 +537,7 @@ JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void
     j.mode = math == MATH_COUNT ? OUT_COUNT : (precision == 1 ? OUT_F64 : OUT_F32);
     int tile[3], region[3];
     choose_grid_tile(fw, n, tu, tile);
-    choose_grid_region(fw, n, tile, tu, region);
+    choose_grid_region(fw, n, tile, tu, region, slice_records);
     long long items = 1;
     for (int d = 0; d < 3; d++) {
         j.n[d] = n[d];
@@ -597,8 +597,6 @@ JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
     return j;
 }
 
-namespace {
-
 int check_grid_dims(const int n[3]) {
     for (int d = 0; d < 3; d++)
         if (n[d] < 2 || n[d] > 65536) {
@@ -613,6 +611,7 @@ int check_grid_dims(const int n[3]) {
     return ST_OK;
 }
 
+namespace {
 
 int persistent_wgs(Context *ctx, int math, long long items) {
     return tiles_workgroups(ctx, math, items, ctx->scratch_wgs);

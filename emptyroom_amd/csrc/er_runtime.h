@@ -161,6 +161,14 @@ int count_incut_grid(Context *ctx, FrameworkDev *fw, const int n[3], unsigned lo
                      uint32_t *per_point_host = nullptr);
 int count_incut_widom(Context *ctx, FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
                       unsigned long long *out);
+// The grid of grid_compute written into caller-owned device memory d_out
+// (n[0] n[1] n[2] floats or doubles), with no device allocation: the work
+// counter, the job descriptor and the region sub-lists live in a fixed
+// device array of the library (er_noalloc.hip).  Same values as
+// grid_compute bit for bit.  Calls that overlap on one device race.
+int grid_compute_noalloc(Context *ctx, FrameworkDev *fw, const int n[3], int precision,
+                         void *d_out);
+int check_grid_dims(const int n[3]);
 // Hard-sphere accessibility grid (src/computegridsheet.h:101-165): 1 where no
 // atom image with offsets in [-1, 1]^3 lies closer than sigma_ij, else 0;
 // grid points f = i / (N - 1), layout k + j*nz + i*ny*nz.
@@ -188,12 +196,15 @@ struct WidomBuffers {
 BinJob make_bin_job(const WidomBuffers &wb, uint64_t n_ins, uint64_t seed, uint32_t stream_id);
 void choose_widom_bins(const FrameworkDev *fw, uint64_t n_ins, const Tuning &tu, int nb[3]);
 void choose_grid_tile(const FrameworkDev *fw, const int n[3], const Tuning &tu, int tile[3]);
+// slice_records: records of one workgroup's region scratch slice (regions
+// are sized so their expected sub-list fills at most 3/4 of a slice).
 void choose_grid_region(const FrameworkDev *fw, const int n[3], const int tile[3],
-                        const Tuning &tu, int region[3]);
+                        const Tuning &tu, int region[3], int slice_records = kRegionRecords);
 JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void *d_out,
-                     const Tuning &tu, int math);
+                     const Tuning &tu, int math, int slice_records = kRegionRecords);
 void choose_region(const FrameworkDev *fw, const double cell_frac[3], const int ncell[3],
-                   const Tuning &tu, long long min_items, int region[3]);
+                   const Tuning &tu, long long min_items, int region[3],
+                   int slice_records = kRegionRecords);
 // min_items: regions of bins are shrunk until the job has at least this many
 // work items (kMinGridItems for a job launched alone).  temperature > 0 lets
 // the tile kernel stop an insertion once its energy is certain to reach

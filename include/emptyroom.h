@@ -138,6 +138,18 @@ int er_framework_release(er_framework *fw);
  * FLT_MAX are stored as FLT_MAX. */
 int er_grid_compute(er_context *ctx, er_framework *fw, const int n[3], int precision,
                     void *host_out, er_grid **resident_out);
+/* No counterpart in the reference: er_grid_compute for callers that cannot
+ * allocate device memory (code under stream capture, tight serving loops,
+ * harnesses that forbid hipMalloc).  The grid (n[0] n[1] n[2] floats for
+ * ER_FP32, doubles for ER_FP64, layout k + j*N_z + i*N_y*N_z) is written into
+ * caller-owned device memory device_out; the call allocates nothing: the
+ * work counter, job descriptor and region sub-lists live in a fixed 32 MiB
+ * device array of the library (one per device, held until the process ends;
+ * no input size needs more).  Values are er_grid_compute's bit for bit.  The
+ * call returns when the grid is written.  Calls that overlap on one device
+ * (other threads or contexts) share that array and race. */
+int er_grid_compute_noalloc(er_context *ctx, er_framework *fw, const int n[3], int precision,
+                            void *device_out);
 int er_grid_download(er_grid *grid, void *host_out);
 int er_grid_info(const er_grid *grid, int n[3], int *precision);
 int er_grid_release(er_grid *grid);
