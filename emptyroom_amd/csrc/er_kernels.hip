@@ -500,6 +500,45 @@ __device__ __forceinline__ void pair_r2(const double4 &A, const double (&mx)[PPL
     }
 }
 
+// Shell records [sb, se) of a segment (record k at Ls[-k]) against this
+// wave's sphere wsph (centre relative to the tile centre, radius wsph.w):
+// records beyond r_c + w of it are out of the cutoff for all of the wave's
+// points and dropped, those within r_c - w are inside it for all of them (no
+// cutoff test).  The wave writes its own list of LDS byte offsets (from the
+// start of the list, capA - 1 - k records), order kept: the nwf wave-far ones
+// from the front of wl, the nws still-shell ones from wl[wcap - 1] down
+// (se - sb <= wcap entries).
+__device__ __forceinline__ void wave_shell_lists(const JobDev *__restrict__ J,
+                                                 const double4 *__restrict__ Ls, int capA,
+                                                 int wcap, int sb, int se, const double4 &wsph,
+                                                 uint16_t *__restrict__ wl, int &nwf, int &nws) {
+    const int lane = threadIdx.x & 63;
+    const double rin = J->rc + wsph.w + kMargin, rfar = J->rc - wsph.w - kMargin;
+    const double in2 = rin * rin, far2 = rfar > 0.0 ? rfar * rfar : -1.0;
+    nwf = 0;
+    nws = 0;
+    for (int k0 = sb; k0 < se; k0 += 64) {
+        const int k = k0 + lane;
+        int cat = 0;
+        if (k < se) {
+            const double4 A = Ls[-k];
+            const double dx = A.x - wsph.x, dy = A.y - wsph.y, dz = A.z - wsph.z;
+            const double d2 = dx * dx + dy * dy + dz * dz;
+            cat = d2 < in2 ? (d2 < far2 ? 1 : 2) : 0;
+        }
+        const uint64_t bf = __ballot(cat == 1), bs = __ballot(cat == 2);
+        const uint16_t off = (uint16_t)((capA - 1 - k) * (int)sizeof(double4));
+        if (cat == 1) wl[nwf + lanes_below(bf)] = off;
+        if (cat == 2) wl[wcap - 1 - (nws + lanes_below(bs))] = off;
+        nwf = uniform(nwf + __popcll(bf));
+        nws = uniform(nws + __popcll(bs));
+    }
+    // the lists are read back by every lane of this wave
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
 // Energies (or in-cutoff counts) of this lane's PPL points against the LDS
 // list.  Points and records are relative to the tile centre c.
 //   Far pairs (inside r_c and beyond the FP64 switch radius for every point
@@ -514,10 +553,10 @@ __device__ __forceinline__ void pair_r2(const double4 &A, const double (&mx)[PPL
 //   reference's clamp and strict cutoff test, FP64 LJ with a Newton-refined
 //   reciprocal (one step for FP32 grids and Widom, two for FP64 grids).
 // XL: see pair_r2.
-// ws (MIXED only, wave-uniform): this wave's points lie in the sphere wsph
-// (centre relative to the tile centre, radius w), smaller than the tile's;
-// each segment's shell records are then re-classified against it into the
-// wave's own lists of LDS byte offsets wl (see below).
+// ws (MIXED and FP64 grid tiles, wave-uniform): this wave's points lie in the
+// sphere wsph (centre relative to the tile centre, radius w), smaller than
+// the tile's; each segment's shell records are then re-classified against it
+// into the wave's own lists of LDS byte offsets wl (wave_shell_lists).
 // phase (wave-uniform): 0 every record, per segment far, shell, near; 1 only
 // the near records; 2 only the far and shell records.
 template <int PPL, int MATH, bool XL>
@@ -615,36 +654,8 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             }
             if (!do_fs) {
             } else if (ws) {
-                // Shell records beyond r_c + w of the wave's sphere are out of
-                // the cutoff for all of the wave's points, and those within
-                // r_c - w are inside it for all of them (no cutoff test); the
-                // wave sorts the segment's shell records into its own list of
-                // LDS byte offsets, order kept: wave-far ones from the front,
-                // still-shell ones from the back.
-                const int lane = threadIdx.x & 63;
-                const double rin = J->rc + wsph.w + kMargin, rfar = J->rc - wsph.w - kMargin;
-                const double in2 = rin * rin, far2 = rfar > 0.0 ? rfar * rfar : -1.0;
-                int nwf = 0, nws = 0;
-                for (int k0 = sb; k0 < se; k0 += 64) {
-                    const int k = k0 + lane;
-                    int cat = 0;
-                    if (k < se) {
-                        const double4 A = Ls[-k];
-                        const double dx = A.x - wsph.x, dy = A.y - wsph.y, dz = A.z - wsph.z;
-                        const double d2 = dx * dx + dy * dy + dz * dz;
-                        cat = d2 < in2 ? (d2 < far2 ? 1 : 2) : 0;
-                    }
-                    const uint64_t bf = __ballot(cat == 1), bs = __ballot(cat == 2);
-                    const uint16_t off = (uint16_t)((capA - 1 - k) * (int)sizeof(double4));
-                    if (cat == 1) wl[nwf + lanes_below(bf)] = off;
-                    if (cat == 2) wl[capA - 1 - (nws + lanes_below(bs))] = off;
-                    nwf = uniform(nwf + __popcll(bf));
-                    nws = uniform(nws + __popcll(bs));
-                }
-                // the lists are read back by every lane of this wave
-                __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-                __builtin_amdgcn_wave_barrier();
-                __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+                int nwf, nws;
+                wave_shell_lists(J, Ls, capA, capA, sb, se, wsph, wl, nwf, nws);
                 const char *__restrict__ Lb = reinterpret_cast<const char *>(L);
                 auto rec = [&](int e) {
                     return *reinterpret_cast<const double4 *>(Lb + wl[e]);
@@ -760,7 +771,25 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
         using no = std::integral_constant<bool, false>;
         using yes = std::integral_constant<bool, true>;
         for (int k = fb; k < fe; ++k) fp64_pair(L[k], no(), no());
-        for (int k = sb; k < se; ++k) fp64_pair(Ls[-k], no(), yes());
+        // wave lists of an FP64 tile hold as many entries as an FP32 tile's
+        // (the far + shell part of the FP32 list), which keeps four
+        // workgroups per CU; a segment with more shell records than that
+        // takes the tile-level path
+        const int wcap = mixed_far_shell_cap(cap);
+        if (ws && se - sb <= wcap) {
+            // shell records against the wave's own sphere (as for FP32
+            // grids): wave-far ones need no cutoff test, those beyond r_c of
+            // every point of the wave contribute exactly 0 and are skipped
+            int nwf, nws;
+            wave_shell_lists(J, Ls, capA, wcap, sb, se, wsph, wl, nwf, nws);
+            const char *__restrict__ Lb = reinterpret_cast<const char *>(L);
+            for (int e = 0; e < nwf; ++e)
+                fp64_pair(*reinterpret_cast<const double4 *>(Lb + wl[e]), no(), no());
+            for (int e = 0; e < nws; ++e)
+                fp64_pair(*reinterpret_cast<const double4 *>(Lb + wl[wcap - 1 - e]), no(), yes());
+        } else {
+            for (int k = sb; k < se; ++k) fp64_pair(Ls[-k], no(), yes());
+        }
         // Near records (an atom within R of the tile centre, where a grid
         // point may sit on the atom): r^2 from the FP64 differences, as
         // src/computegridsheet.h:71-75 forms it, so that a point on an atom
@@ -810,14 +839,15 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
     long long oidx[PPL];
     bool ok[PPL];
     const int t0 = J->tile[0], t1 = J->tile[1], t2 = J->tile[2];
-    // Quadrant layout (FP32 grids, even t1 and t2, t0 divisible by PPL): wave w
+    // Quadrant layout (FP32 and FP64 grids, even t1 and t2, t0 divisible by PPL): wave w
     // owns the (j, k) quadrant (w & 1, w >> 1) of the tile and all of its i;
     // lane l holds points i = il + q * t0 / PPL.  Otherwise point p = tid +
     // 256 q of the tile in k-fastest order.
     const int wave = tid >> 6, lane = tid & 63;
     // (the wave lists hold 16-bit LDS byte offsets into the far/shell part)
-    const bool quad = MATH == MATH_MIXED && t1 % 2 == 0 && t2 % 2 == 0 && t0 % PPL == 0 &&
-                      mixed_far_shell_cap(cap) * (int)sizeof(double4) <= 65536;
+    const bool quad = (MATH == MATH_MIXED || MATH == MATH_FP64) && t1 % 2 == 0 &&
+                      t2 % 2 == 0 && t0 % PPL == 0 &&
+                      near_region_start<MATH>(cap) * (int)sizeof(double4) <= 65536;
     const int h1 = t1 / 2, h2 = t2 / 2;
 #pragma unroll
     for (int q = 0; q < PPL; ++q) {
@@ -959,8 +989,8 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
     const int tid = threadIdx.x;
     double4 *__restrict__ slice =
         scratch ? scratch + (size_t)blockIdx.x * (size_t)scratch_cap : nullptr;
-    // per-wave shell offset lists after the LDS list (FP32 grids only)
-    uint16_t *__restrict__ s_wl = MATH == MATH_MIXED
+    // per-wave shell offset lists after the LDS list (FP32 and FP64 grids)
+    uint16_t *__restrict__ s_wl = (MATH == MATH_MIXED || MATH == MATH_FP64)
         ? reinterpret_cast<uint16_t *>(s_list + cap) + (tid >> 6) * mixed_far_shell_cap(cap)
         : nullptr;
 
@@ -1363,7 +1393,8 @@ hipError_t launch_bin_all(const BinJob *jobs, int njobs, long long total_blocks,
 
 size_t tiles_lds_bytes(int cap, int math) {
     size_t b = (size_t)cap * sizeof(double4);
-    if (math == MATH_MIXED) b += (size_t)(kWG / 64) * mixed_far_shell_cap(cap) * sizeof(uint16_t);
+    if (math == MATH_MIXED || math == MATH_FP64)
+        b += (size_t)(kWG / 64) * mixed_far_shell_cap(cap) * sizeof(uint16_t);
     return (b + 15) & ~(size_t)15;
 }
 
