@@ -208,17 +208,22 @@ def percentile(v, q):
     return float(np.percentile(np.asarray(v), q)) if len(v) else None
 
 
-def serve_rate(api, synth, ctx, parts, rate, seconds, streams, items_per_launch):
+def serve_rate(api, synth, ctx, parts, rate, seconds, streams, items_per_launch, cycle=False,
+               cache_bytes=-1):
     """One Poisson stream at `rate` requests/s for about `seconds` into a fresh
     server on ctx: grid / Henry requests on frameworks drawn (seeded) from the
-    pool; latency = completion - submission on the server's clock; grids stay
-    resident in HBM and are released unread."""
+    pool, or taken in turn with cycle; latency = completion - submission on the
+    server's clock; grids stay resident in HBM and are released unread."""
     import threading
     n_req = max(50, int(rate * seconds))
     arrivals, kinds, temps = synth.config4_stream(rate, n_req)
-    pick = (synth.Rng(synth.framework_seed(4, 1)).u64(n_req) %
-            np.uint64(len(parts))).astype(np.int64)
-    srv = api.Server(ctx, streams=streams, items_per_launch=items_per_launch)
+    if cycle:
+        pick = np.arange(n_req) % len(parts)
+    else:
+        pick = (synth.Rng(synth.framework_seed(4, 1)).u64(n_req) %
+                np.uint64(len(parts))).astype(np.int64)
+    srv = api.Server(ctx, streams=streams, items_per_launch=items_per_launch,
+                     cache_bytes=cache_bytes)
     tickets, codes = [], []
 
     def submit_all():
@@ -276,7 +281,8 @@ def bench_serving(args, api, synth):
     rows = []
     for rate in rates:
         row = serve_rate(api, synth, ctx, parts, rate, args.serve_seconds, args.streams,
-                         args.items_per_launch)
+                         args.items_per_launch, cycle=args.pick == "cycle",
+                         cache_bytes=args.cache_bytes)
         log(f"serving rate {rate:g}/s: {json.dumps(row)}")
         rows.append(row)
     ctx.close()
@@ -292,7 +298,8 @@ def bench_serving(args, api, synth):
                                "drawn from a pool of the batching mix, T ~ U[250,350] K, "
                                "HBM cap 200 GB",
                    "frameworks_in_pool": len(pool), "streams": args.streams,
-                   "items_per_launch": args.items_per_launch,
+                   "items_per_launch": args.items_per_launch, "pick": args.pick,
+                   "cache_bytes": args.cache_bytes,
                    "seconds_per_rate": args.serve_seconds, "parallelism": "single GPU"},
         "rates": rows,
     }
@@ -497,6 +504,10 @@ def main():
     ap.add_argument("--serve-rate", type=float, default=500.0,
                     help="arrival rate of the serving leg of the default run (req/s)")
     ap.add_argument("--serve-leg-seconds", type=float, default=2.0)
+    ap.add_argument("--pick", default="random", choices=["random", "cycle"],
+                    help="config4: frameworks drawn from the pool at random or in turn")
+    ap.add_argument("--cache-bytes", type=int, default=-1,
+                    help="config4: idle frameworks kept resident (-1 up to the budget, 0 none)")
     ap.add_argument("--streams", type=int, default=3)
     ap.add_argument("--items-per-launch", type=int, default=32768)
     ap.add_argument("--tuning-file", default=DEFAULT_TUNING_FILE,
