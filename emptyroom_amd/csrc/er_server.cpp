@@ -18,13 +18,24 @@ double now_s() {
 
 constexpr int kMaxLaunchJobs = 1024;
 
-// FNV-1a over the bytes that define an uploaded framework.
+// Hash of the bytes that define an uploaded framework: 64-bit words mixed
+// with a multiply-xorshift step (a hit is confirmed by a full comparison).
 uint64_t hash_bytes(uint64_t h, const void *p, size_t n) {
     const unsigned char *b = static_cast<const unsigned char *>(p);
-    for (size_t i = 0; i < n; i++) {
-        h ^= b[i];
-        h *= 0x100000001b3ULL;
+    auto mix = [&h](uint64_t w) {
+        h ^= w;
+        h *= 0x9E3779B97F4A7C15ULL;
+        h ^= h >> 29;
+    };
+    size_t i = 0;
+    for (; i + 8 <= n; i += 8) {
+        uint64_t w;
+        std::memcpy(&w, b + i, 8);
+        mix(w);
     }
+    uint64_t t = 0;
+    std::memcpy(&t, b + i, n - i);
+    mix(t ^ ((uint64_t)n << 56));
     return h;
 }
 
@@ -190,8 +201,7 @@ Server::FwEntry *Server::cache_find(const Req *r) {
     return nullptr;
 }
 
-Server::FwEntry *Server::fw_acquire(Req *r, int *st) {
-    FwEntry *e = cache_find(r);
+Server::FwEntry *Server::fw_acquire(Req *r, FwEntry *e, int *st) {
     if (e) {
         if (e->refs == 0) idle_bytes_ -= e->bytes;
         e->refs++;
@@ -295,12 +305,12 @@ int Server::submit(const RequestSpec &spec, uint64_t *ticket) {
     return ST_OK;
 }
 
-bool Server::admit_one(Req *r) {
+bool Server::admit_one(Req *r, FwEntry *hit) {
     // framework (resident or uploaded), outputs and Widom binning, on the
     // context stream
     const RequestSpec &s = r->spec;
     int st = ST_OK;
-    r->fwe = fw_acquire(r, &st);
+    r->fwe = fw_acquire(r, hit, &st);
     if (r->fwe) r->fw = r->fwe->fw;
     const bool grid = s.grid_n[0] || s.grid_n[1] || s.grid_n[2];
     if (st == ST_OK && grid) {
@@ -472,13 +482,13 @@ void Server::run() {
         //    resident framework costs nothing, idle ones are evicted for room
         while (!queue_.empty()) {
             Req *r = queue_.front();
-            const FwEntry *hit = cache_find(r);
+            FwEntry *hit = cache_find(r);
             const size_t need = r->work_reserve + (hit ? 0 : r->fw_reserve);
             bool fits = ctx_->pool.in_use() + need <= cap_;
             while (!fits && evict_one(hit)) fits = ctx_->pool.in_use() + need <= cap_;
             if (!fits) break;
             queue_.pop_front();
-            if (admit_one(r)) active_.push_back(r);
+            if (admit_one(r, hit)) active_.push_back(r);
             progress = true;
         }
         // 4. fill free launch slots with pending items

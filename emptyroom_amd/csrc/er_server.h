@@ -109,12 +109,12 @@ class Server {
     struct Launch;
     struct FwEntry;
     void run();
-    bool admit_one(Req *r);
+    bool admit_one(Req *r, FwEntry *hit);  // hit: the resident framework, or nullptr
     void finish(Req *r);
     void fail(Req *r);  // state RQ_FAILED (status / error already set); frees its buffers
     void fail_parts(Launch *L, const std::string &why);
     FwEntry *cache_find(const Req *r);
-    FwEntry *fw_acquire(Req *r, int *st);
+    FwEntry *fw_acquire(Req *r, FwEntry *hit, int *st);
     void fw_release(FwEntry *e);
     bool evict_one(const FwEntry *keep);
     void trim_idle();
