@@ -742,12 +742,19 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             continue;
         }
         // FP64 grids: every pair in FP64.  The expanded FP64 r^2 of the near
-        // pairs of FP32 grids, u = 1/r^2 from the FP32 reciprocal and two
-        // Newton steps (FP64 accuracy), FP64 sums of u^3 and u^6 scaled by
-        // 4 eps sigma^6 / sigma^12 per segment.  Far records (inside r_c for
-        // every point, farther than kMargin from all) need neither the clamp
-        // nor the cutoff test, shell records only the test, near records
-        // (an atom within R of the tile centre) both.
+        // pairs of FP32 grids, u = 1/r^2 from the FP32 reciprocal refined in
+        // FP64, FP64 sums of u^3 and u^6 scaled by 4 eps sigma^6 / sigma^12
+        // per segment.  Far records (inside r_c for every point, farther than
+        // kMargin from all) need neither the clamp nor the cutoff test, shell
+        // records only the test, near records (an atom within R of the tile
+        // centre) both.
+        // One Newton step: the FP32 reciprocal of the rounded r^2 is within
+        // e0 <= 2^-24 + 2^-23 of 1/r^2, and y + y (1 - r^2 y) leaves 1/r^2
+        // (1 - e0^2) plus one FP64 rounding, so u is low by at most 3.2e-14
+        // relative and a pair's energy is off by at most 2e-13 of |4 eps
+        // sigma^12 u^6| + |4 eps sigma^6 u^3| -- the size of the rounding the
+        // expanded r^2 already leaves (|a|^2 + |p|^2 - 2 p.a cancels ~13 FP64
+        // ulps of |a|^2).  Near records take two steps.
         const double sig6 = P.sig2 * P.sig2 * P.sig2;
         const double c6 = P.eps4 * sig6 * sig6, c3 = -P.eps4 * sig6;
         double b3[PPL], b6[PPL];
@@ -760,7 +767,6 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             for (int q = 0; q < PPL; ++q) {
                 const double rr = decltype(clamp)::value ? fmax(r2[q], min_r2) : r2[q];  // :77
                 double y = (double)__builtin_amdgcn_rcpf((float)rr);
-                y = fma(fma(-rr, y, 1.0), y, y);
                 y = fma(fma(-rr, y, 1.0), y, y);
                 double y3 = y * y * y;
                 if (decltype(test)::value) y3 = rr < rc2 ? y3 : 0.0;  // src/computegridsheet.h:79
