@@ -44,7 +44,9 @@ typedef enum er_status {
 
 typedef enum er_precision {
     ER_FP32 = 0, /* FP32 grid: FP32 LJ for pairs beyond 1.5 sigma (tunable), FP64 r^2, close pairs and sums */
-    ER_FP64 = 1  /* FP64 grid: every pair in FP64 (the reference's result to ~1e-13 relative) */
+    ER_FP64 = 1  /* FP64 grid: FP64 sums, far pairs from an FP32 reciprocal plus one Newton step
+                    (bound: 2e-13 of the sum of the terms' magnitudes; measured on the batching
+                    mix: within 1.3e-10 of max(1,|E|) of the reference's direct method) */
 } er_precision;
 
 typedef struct er_context er_context;
