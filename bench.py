@@ -246,7 +246,7 @@ def serve_rate(api, synth, ctx, parts, rate, seconds, streams, items_per_launch,
     th = threading.Thread(target=submit_all)
     th.start()
     th.join()
-    lat, t_first, t_last, done = [], None, None, 0
+    lat, wait, run, t_first, t_last, done = [], [], [], None, None, 0
     for code, tk in zip(codes, tickets):
         if code != 0:
             continue
@@ -254,6 +254,8 @@ def serve_rate(api, synth, ctx, parts, rate, seconds, streams, items_per_launch,
         if st.state == api.RQ_DONE:
             done += 1
             lat.append((st.t_done - st.t_submit) * 1e3)
+            wait.append((st.t_admit - st.t_submit) * 1e3)
+            run.append((st.t_done - st.t_admit) * 1e3)
             t_first = st.t_submit if t_first is None else min(t_first, st.t_submit)
             t_last = st.t_done if t_last is None else max(t_last, st.t_done)
         srv.release(tk)
@@ -264,6 +266,8 @@ def serve_rate(api, synth, ctx, parts, rate, seconds, streams, items_per_launch,
     return dict(rate=rate, requests=n_req, completed=done,
                 rejected=stats["rejected"], rejection_rate=stats["rejected"] / n_req,
                 p50_ms=percentile(lat, 50), p99_ms=percentile(lat, 99),
+                admit_wait_p50_ms=percentile(wait, 50), admit_wait_p99_ms=percentile(wait, 99),
+                run_p50_ms=percentile(run, 50), run_p99_ms=percentile(run, 99),
                 sustained_req_per_s=done / span if done else 0.0,
                 sustained_fws_per_s=done / span if done else 0.0,
                 launches=stats["launches"], framework_uploads=cache["misses"],

@@ -102,6 +102,7 @@ struct Server::Req {
     hipEvent_t reduced = nullptr; // Widom sums copied to host
     double *h_sums = nullptr;     // pinned, 2 doubles
     bool reducing = false;
+    bool is_ready = false;        // `ready` seen complete (queried until then only)
 };
 
 struct Server::Launch {
@@ -498,7 +499,7 @@ void Server::run() {
             for (Req *r : active_)
                 if (!r->failed &&
                     (r->grid_next < r->grid_job.n_items || r->widom_next < r->widom_job.n_items) &&
-                    hipEventQuery(r->ready) == hipSuccess)
+                    (r->is_ready || (r->is_ready = hipEventQuery(r->ready) == hipSuccess)))
                     ready.push_back(r);
             if (ready.empty()) break;
             // the pair-math variant of this launch: that of the oldest ready
