@@ -127,7 +127,8 @@ constexpr int kMaxSeg = 64;   // list segments per fill pass
 constexpr int kMaxTypes = 64; // pair types a region sub-list can index
 constexpr int kLdsBinsMax = 32768; // Widom bins a binning block histograms in LDS (128 KiB)
 constexpr int kBinChunk = 16384;  // insertions per binning block
-constexpr int kRegionRecords = 8192;  // records of one workgroup's region sub-list (256 KiB)
+constexpr int kRegionRecords = 8192;  // entries of one workgroup's region sub-list (near-list
+                                      // indices, 32 KiB)
 constexpr long long kMinGridItems = 2048;  // regions per grid before regions shrink
 
 }  // namespace er

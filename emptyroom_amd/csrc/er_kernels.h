@@ -58,11 +58,11 @@ int bin_blocks(uint64_t n, int chunk);
 // Runs the work items [0, total_items) of the jobs (device array) with a
 // persistent grid of `workgroups` blocks pulling items from *counter (which
 // must be zero).  cap = LDS list capacity in records.  scratch (nullable)
-// holds workgroups * scratch_cap records of region sub-lists; without it
-// every tile streams the framework's whole near list.
+// holds workgroups * scratch_cap entries of region sub-lists (near-list
+// indices); without it every tile streams the framework's whole near list.
 hipError_t launch_tiles(int math, int ppl, const JobDev *jobs_dev, int njobs,
                         long long total_items, unsigned long long *counter, int workgroups,
-                        int cap, double4 *scratch, int scratch_cap, hipStream_t s);
+                        int cap, uint32_t *scratch, int scratch_cap, hipStream_t s);
 // LDS bytes the tiled kernel needs for a list capacity.
 size_t tiles_lds_bytes(int cap, int math);
 // Far + shell part of an FP32-grid LDS list of cap records (the rest holds

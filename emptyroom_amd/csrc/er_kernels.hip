@@ -336,11 +336,14 @@ __device__ __forceinline__ int near_region_start(int cap) {
 // but possibly outside the cutoff), near (possibly inside the switch radius).
 // Records hold the atom relative to the tile centre and its squared distance,
 // (a - c, |a - c|^2), and keep the source order, so the list (and every sum
-// over it) depends only on the tile and the source.  Returns the number of
-// segments written to seg.
+// over it) depends only on the tile and the source.  The source is the near
+// list src, position p holding src[p], or, with sidx, a sub-list of it,
+// position p holding src[sidx[p]].  Returns the number of segments written
+// to seg.
 template <int MATH>
 __device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict__ src,
-                         const int *__restrict__ tbeg, const Tile &tl, int &cur_t, int &cur_pos,
+                         const uint32_t *__restrict__ sidx, const int *__restrict__ tbeg,
+                         const Tile &tl, int &cur_t, int &cur_pos,
                          int cap, double4 *__restrict__ L, int *__restrict__ seg,
                          int *__restrict__ s_cnt) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
@@ -389,7 +392,7 @@ __device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict
         int cat = CAT_NONE;
         double4 A = make_double4(0.0, 0.0, 0.0, 0.0);
         if (idx < tend) {
-            const double4 S = src[idx];
+            const double4 S = src[sidx ? (int)sidx[idx] : idx];
             const double dx = S.x - tl.cx, dy = S.y - tl.cy, dz = S.z - tl.cz;
             const double d2 = dx * dx + dy * dy + dz * dz;
             A = make_double4(dx, dy, dz, d2);
@@ -825,7 +828,8 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
 // src / tbeg (the framework's near list, or a region's sub-list of it).
 template <int PPL, int MATH>
 __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
-                          const double4 *__restrict__ src, const int *__restrict__ tbeg, int cap,
+                          const double4 *__restrict__ src, const uint32_t *__restrict__ sidx,
+                          const int *__restrict__ tbeg, int cap,
                           double4 *__restrict__ s_list, int *__restrict__ s_seg,
                           int *__restrict__ s_cnt, uint16_t *__restrict__ s_wl) {
     const int tid = threadIdx.x;
@@ -907,7 +911,7 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
     int cur_t = 0, cur_pos = uniform(tbeg[0]);
     do {
         const int nseg =
-            fill_list<MATH>(J, src, tbeg, tl, cur_t, cur_pos, cap, s_list, s_seg, s_cnt);
+            fill_list<MATH>(J, src, sidx, tbeg, tl, cur_t, cur_pos, cap, s_list, s_seg, s_cnt);
         __syncthreads();
         if (!idle) {
             if (xl)
@@ -940,9 +944,13 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
 // region's tiles.  Any atom a tile's own fill would keep (d_t - R_t < r_c)
 // therefore lies in the sub-list, and the sub-list keeps the near-list order,
 // so every tile builds exactly the list it would build from the near list.
-// Returns false (and builds nothing usable) if the sub-list exceeds cap_s.
+// The sub-list holds near-list indices (4 bytes a record): the records
+// themselves are read from the near list, which every workgroup on a
+// framework shares (it stays in L2), while each workgroup's sub-list is its
+// own.  Returns false (and builds nothing usable) if the sub-list exceeds
+// cap_s.
 __device__ bool build_region_list(const JobDev *__restrict__ J, double cx, double cy, double cz,
-                                  double in2, double4 *__restrict__ out, int cap_s,
+                                  double in2, uint32_t *__restrict__ out, int cap_s,
                                   int *__restrict__ rbeg, int *__restrict__ s_cnt) {
     const int tid = threadIdx.x, lane = tid & 63, wave = tid >> 6;
     int n = 0;
@@ -969,7 +977,7 @@ __device__ bool build_region_list(const JobDev *__restrict__ J, double cx, doubl
             }
             __syncthreads();
             if (n + tot > cap_s) return false;  // uniform
-            if (keep) out[off + lanes_below(bk)] = A;
+            if (keep) out[off + lanes_below(bk)] = (uint32_t)idx;
             n = uniform(n + tot);
         }
     }
@@ -984,7 +992,7 @@ template <int PPL, int MATH>
 __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, int njobs,
                                                long long total_items,
                                                unsigned long long *__restrict__ counter,
-                                               int cap, double4 *__restrict__ scratch,
+                                               int cap, uint32_t *__restrict__ scratch,
                                                int scratch_cap) {
     extern __shared__ double4 s_list[];
     __shared__ int s_seg[kSegInts * kMaxSeg];
@@ -993,7 +1001,7 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
     __shared__ double s_red[kWG / 64];
     __shared__ long long s_item;
     const int tid = threadIdx.x;
-    double4 *__restrict__ slice =
+    uint32_t *__restrict__ slice =
         scratch ? scratch + (size_t)blockIdx.x * (size_t)scratch_cap : nullptr;
     // per-wave shell offset lists after the LDS list (FP32 and FP64 grids)
     uint16_t *__restrict__ s_wl = (MATH == MATH_MIXED || MATH == MATH_FP64)
@@ -1055,6 +1063,7 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             tc[2] = t0[2] + ti % tn[2];
         };
         const double4 *src = J->near;
+        const uint32_t *sidx = nullptr;
         const int *tbeg = J->type_begin;
         if (nt > 1 && slice && J->ntypes <= kMaxTypes) {
             // region centre: middle of the region's box
@@ -1092,7 +1101,7 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             __syncthreads();
             const double rr = J->rc + R + kMargin;
             if (build_region_list(J, cx, cy, cz, rr * rr, slice, scratch_cap, s_rbeg, s_cnt)) {
-                src = slice;
+                sidx = slice;
                 tbeg = s_rbeg;
             }
         }
@@ -1100,7 +1109,7 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             int tc[3];
             cell_of(ti, tc);
             if (grid) {
-                grid_tile<PPL, MATH>(J, tc, src, tbeg, cap, s_list, s_seg, s_cnt, s_wl);
+                grid_tile<PPL, MATH>(J, tc, src, sidx, tbeg, cap, s_list, s_seg, s_cnt, s_wl);
                 continue;
             }
             // ---- one Widom bin: every insertion whose point falls in it ----
@@ -1168,7 +1177,7 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             int cur_t = 0, cur_pos = uniform(tbeg[0]);
             bool first = true;
             do {
-                const int nseg = fill_list<MATH>(J, src, tbeg, tl, cur_t, cur_pos, cap, s_list,
+                const int nseg = fill_list<MATH>(J, src, sidx, tbeg, tl, cur_t, cur_pos, cap, s_list,
                                                  s_seg, s_cnt);
                 __syncthreads();
                 // -eps_ij for every record of this pass (near; far and shell)
@@ -1317,7 +1326,7 @@ int grid_blocks(long long n, int per_block, int cap_blocks) {
 
 template <int PPL, int MATH>
 hipError_t launch_tiles_t(const JobDev *jobs, int njobs, long long total, unsigned long long *ctr,
-                          int wgs, int cap, double4 *scratch, int scratch_cap, hipStream_t s) {
+                          int wgs, int cap, uint32_t *scratch, int scratch_cap, hipStream_t s) {
     hipLaunchKernelGGL((k_tiles<PPL, MATH>), dim3(wgs), dim3(kWG), tiles_lds_bytes(cap, MATH), s, jobs,
                        njobs, total, ctr, cap, scratch, scratch_cap);
     return hipGetLastError();
@@ -1405,7 +1414,7 @@ size_t tiles_lds_bytes(int cap, int math) {
 }
 
 hipError_t launch_tiles(int math, int ppl, const JobDev *jobs, int njobs, long long total,
-                        unsigned long long *ctr, int wgs, int cap, double4 *scratch,
+                        unsigned long long *ctr, int wgs, int cap, uint32_t *scratch,
                         int scratch_cap, hipStream_t s) {
     if (total <= 0) return hipSuccess;
     if (cap < 2 * kWG || wgs <= 0) return hipErrorInvalidValue;

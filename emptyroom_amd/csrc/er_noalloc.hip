@@ -9,7 +9,8 @@
 //                     fill kernel before every launch)
 //   [256, 4096)       the job descriptor
 //   [4096, ...)       region sub-lists: kNoAllocSlices slices of
-//                     kNoAllocRecords records (32 B), one per workgroup
+//                     kNoAllocRecords near-list indices (4 B), one per
+//                     workgroup
 // 32 MiB + 4 KiB in all.  No input size changes what a call needs: regions
 // are sized to fit a slice (a region whose sub-list still overflows streams
 // the framework's whole near list instead), and a launch uses at most
@@ -28,11 +29,11 @@ namespace er {
 namespace {
 
 constexpr int kNoAllocSlices = 1024;
-constexpr int kNoAllocRecords = 1024;
+constexpr int kNoAllocRecords = 8192;
 constexpr size_t kNoAllocJobOffset = 256;
 constexpr size_t kNoAllocListOffset = 4096;
 constexpr size_t kNoAllocBytes =
-    kNoAllocListOffset + (size_t)kNoAllocSlices * kNoAllocRecords * sizeof(double4);
+    kNoAllocListOffset + (size_t)kNoAllocSlices * kNoAllocRecords * sizeof(uint32_t);
 static_assert(kNoAllocJobOffset + sizeof(JobDev) <= kNoAllocListOffset,
               "job descriptor does not fit its scratch slot");
 
@@ -97,7 +98,7 @@ int grid_compute_noalloc(Context *ctx, FrameworkDev *fw, const int n[3], int pre
     if ((st = scratch_address(ctx->device, &base)) != ST_OK) return st;
     unsigned long long *counter = reinterpret_cast<unsigned long long *>(base);
     JobDev *d_job = reinterpret_cast<JobDev *>(base + kNoAllocJobOffset);
-    double4 *slices = reinterpret_cast<double4 *>(base + kNoAllocListOffset);
+    uint32_t *slices = reinterpret_cast<uint32_t *>(base + kNoAllocListOffset);
 
     const int math = precision == 1 ? MATH_FP64 : MATH_MIXED;
     // regions sized for this array's slices; a tile's list, and so every

@@ -228,7 +228,7 @@ int tiles_workgroups(const Context *ctx, int math, long long items, int scratch_
     return (int)std::max<long long>(w, 1);
 }
 
-int scratch_ensure(Context *ctx, hipStream_t s, double4 **buf, int *wgs) {
+int scratch_ensure(Context *ctx, hipStream_t s, uint32_t **buf, int *wgs) {
     int occ = 1;
     for (int m : {(int)MATH_MIXED, (int)MATH_FP64, (int)MATH_COUNT, (int)MATH_OVERLAP})
         occ = std::max(occ, tiles_occupancy(m, ctx->tuning.ppl, ctx->tuning.cap));
@@ -238,7 +238,7 @@ int scratch_ensure(Context *ctx, hipStream_t s, double4 **buf, int *wgs) {
     // workgroups, which changes no result.
     void *p = nullptr;
     for (; need >= 1 && need > *wgs; need /= 2) {
-        p = ctx->pool.alloc((size_t)need * kRegionRecords * sizeof(double4));
+        p = ctx->pool.alloc((size_t)need * kRegionRecords * sizeof(uint32_t));
         if (p) break;
     }
     if (!p) {
@@ -250,7 +250,7 @@ int scratch_ensure(Context *ctx, hipStream_t s, double4 **buf, int *wgs) {
         HIP_TRY(hipStreamSynchronize(s));
         ctx->pool.free(*buf);
     }
-    *buf = (double4 *)p;
+    *buf = (uint32_t *)p;
     *wgs = need;
     return ST_OK;
 }

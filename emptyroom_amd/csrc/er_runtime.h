@@ -96,7 +96,7 @@ class Context {
     unsigned long long *d_counter = nullptr;  // work counter of plain launches
     double2 *d_partials = nullptr;
     double *d_sums = nullptr;
-    double4 *d_scratch = nullptr;  // region sub-lists, scratch_wgs * kRegionRecords records
+    uint32_t *d_scratch = nullptr;  // region sub-lists, scratch_wgs * kRegionRecords indices
     int scratch_wgs = 0;
     void *h_stage = nullptr;       // pinned host staging of pocket labelling results
     size_t h_stage_bytes = 0;
@@ -113,7 +113,7 @@ int tiles_workgroups(const Context *ctx, int math, long long items, int scratch_
 // Region scratch for every workgroup the current tuning can keep resident
 // (n_cu * occupancy); grows *buf if needed.  Called at context creation and
 // after a tuning change; never during graph capture.
-int scratch_ensure(Context *ctx, hipStream_t s, double4 **buf, int *wgs);
+int scratch_ensure(Context *ctx, hipStream_t s, uint32_t **buf, int *wgs);
 
 struct FrameworkDev {
     Context *ctx = nullptr;

@@ -111,7 +111,7 @@ struct Server::Launch {
     JobDev *h_jobs = nullptr;     // pinned
     JobDev *d_jobs = nullptr;
     unsigned long long *d_counter = nullptr;
-    double4 *d_scratch = nullptr;  // region sub-lists of this launch slot
+    uint32_t *d_scratch = nullptr;  // region sub-lists of this launch slot
     int scratch_wgs = 0;
     std::vector<std::pair<Req *, long long>> parts;
     bool busy = false;
