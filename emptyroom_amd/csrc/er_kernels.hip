@@ -1168,12 +1168,79 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                 for (int q = 0; q < P; ++q)
                     if (ok[q]) J->e_out[ins[q]] = first ? acc[q] : prev[q] + acc[q];
             };
+            // A bin whose insertions fit one chunk (at most PW * 256, most
+            // bins) keeps each lane's points and partial energies in
+            // registers across the passes and writes the energies once; the
+            // passes add the same terms in the same order as the chunked
+            // form below, which stores and reloads them after every pass.
+            auto resident = [&](auto pw, const double (&X)[PW], const double (&Y)[PW],
+                                const double (&Z)[PW], const bool (&OK)[PW], double (&T)[PW],
+                                const Tile &tl, int nseg, bool first, double lb_near,
+                                double lb_fs, double lb_later) {
+                constexpr int P = decltype(pw)::value;
+                double px[P], py[P], pz[P], acc[P], prev[P];
+#pragma unroll
+                for (int q = 0; q < P; ++q) {
+                    px[q] = X[q];
+                    py[q] = Y[q];
+                    pz[q] = Z[q];
+                    acc[q] = 0.0;
+                    prev[q] = first ? 0.0 : T[q];
+                }
+                const bool sat = J->sat_e > 0.0;
+                auto saturated = [&](double lb) {
+                    bool all = true;
+#pragma unroll
+                    for (int q = 0; q < P; ++q)
+                        if (OK[q]) all = all && prev[q] + acc[q] + lb >= J->sat_e;
+                    return __all(all) != 0;
+                };
+                if (sat && !first && saturated(lb_near + lb_fs + lb_later)) return;
+                eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
+                                          false, make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
+                if (!(sat && saturated(lb_fs + lb_later)))
+                    eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc,
+                                              tl.R, false, make_double4(0.0, 0.0, 0.0, 0.0),
+                                              nullptr, 2);
+#pragma unroll
+                for (int q = 0; q < P; ++q) T[q] = first ? acc[q] : prev[q] + acc[q];
+            };
             const int b = (tc[0] * J->nbins[1] + tc[1]) * J->nbins[2] + tc[2];
             const uint32_t s0 = J->bin_start[b], s1 = J->bin_start[b + 1];
             if (s0 == s1) continue;
             double lo[3], hi[3];
             cell_box(tc, lo, hi);
             const Tile tl = tile_sphere(J->t, lo, hi);
+            // one chunk: this wave's points per lane (wave-uniform; 0 for a
+            // wave with no slot); several chunks: -1.  Only the FP32-grid /
+            // Widom pair math keeps bins resident: Widom jobs never run with
+            // the other variants, whose registers the resident state would
+            // only crowd (FP64 grids would drop to three waves per SIMD).
+            const uint64_t wv0 = s0 + (uint64_t)(tid & ~63);
+            const int mode = (MATH != MATH_MIXED || s1 - s0 > (uint32_t)(PW * kWG)) ? -1
+                             : wv0 >= s1                     ? 0
+                             : (PW > 1 && wv0 + kWG >= s1)   ? 1
+                                                             : PW;
+            double rx[PW], ry[PW], rz[PW], rt[PW];
+            uint32_t rins[PW];
+            bool rok[PW];
+#pragma unroll
+            for (int q = 0; q < PW; ++q) {
+                rx[q] = ry[q] = rz[q] = rt[q] = 0.0;
+                rins[q] = 0;
+                rok[q] = false;
+                if (q < mode) {
+                    const uint64_t p = s0 + (uint64_t)(q * kWG + tid);
+                    rok[q] = p < s1;
+                    rins[q] = J->sorted[rok[q] ? p : s0];
+                    double f[3];
+                    widom_point(J->seed, J->stream, rins[q], f);
+                    frac_to_cart(J->t, f[0], f[1], f[2], rx[q], ry[q], rz[q]);
+                    rx[q] -= tl.cx;
+                    ry[q] -= tl.cy;
+                    rz[q] -= tl.cz;
+                }
+            }
             int cur_t = 0, cur_pos = uniform(tbeg[0]);
             bool first = true;
             do {
@@ -1194,7 +1261,13 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                     if (cur_t < J->ntypes)
                         lb_later = -J->eps_max * (double)(uniform(tbeg[J->ntypes]) - cur_pos);
                 }
-                for (uint64_t c = s0; c < s1; c += (uint64_t)(PW * kWG)) {
+                if (mode == PW)
+                    resident(std::integral_constant<int, PW>(), rx, ry, rz, rok, rt, tl, nseg,
+                             first, lb_near, lb_fs, lb_later);
+                else if (PW > 1 && mode == 1)
+                    resident(std::integral_constant<int, 1>(), rx, ry, rz, rok, rt, tl, nseg,
+                             first, lb_near, lb_fs, lb_later);
+                for (uint64_t c = s0; mode < 0 && c < s1; c += (uint64_t)(PW * kWG)) {
                     const uint64_t w0 = c + (uint64_t)(tid & ~63);  // wave-uniform
                     if (w0 >= s1) continue;
                     if (PW > 1 && w0 + kWG >= s1)
@@ -1207,6 +1280,9 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                 first = false;
                 __syncthreads();
             } while (cur_t < J->ntypes);
+#pragma unroll
+            for (int q = 0; q < PW; ++q)
+                if (q < mode && rok[q]) J->e_out[rins[q]] = rt[q];
         }
     }
 }
