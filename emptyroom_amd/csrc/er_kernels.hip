@@ -507,17 +507,25 @@ __device__ __forceinline__ void pair_r2(const double4 &A, const double (&mx)[PPL
 // wave's sphere wsph (centre relative to the tile centre, radius wsph.w):
 // records beyond r_c + w of it are out of the cutoff for all of the wave's
 // points and dropped, those within r_c - w are inside it for all of them (no
-// cutoff test).  The wave writes its own list of LDS byte offsets (from the
-// start of the list, capA - 1 - k records), order kept: the nwf wave-far ones
-// from the front of wl, the nws still-shell ones from wl[wcap - 1] down
-// (se - sb <= wcap entries).
+// cutoff test).  The wave's points fill a parallelepiped around that centre,
+// wbox.{x,y,z} times the cell vectors on either side; for a record at v from
+// the centre every point x has |a - x| >= (a - x).v/|v| >= |v| - h(v)/|v|,
+// h(v) = sum_d wbox_d |t_d . v| (t_d the cell vectors), so a record with
+// |v|^2 - h(v) > (r_c + m)|v| is out of the cutoff for every point as well
+// and is dropped: it added exactly 0 to every sum.  The wave writes its own
+// list of LDS byte offsets (from the start of the list, capA - 1 - k
+// records), order kept: the nwf wave-far ones from the front of wl, the nws
+// still-shell ones from wl[wcap - 1] down (se - sb <= wcap entries).
 __device__ __forceinline__ void wave_shell_lists(const JobDev *__restrict__ J,
                                                  const double4 *__restrict__ Ls, int capA,
                                                  int wcap, int sb, int se, const double4 &wsph,
-                                                 uint16_t *__restrict__ wl, int &nwf, int &nws) {
+                                                 const double4 &wbox, uint16_t *__restrict__ wl,
+                                                 int &nwf, int &nws) {
     const int lane = threadIdx.x & 63;
     const double rin = J->rc + wsph.w + kMargin, rfar = J->rc - wsph.w - kMargin;
     const double in2 = rin * rin, far2 = rfar > 0.0 ? rfar * rfar : -1.0;
+    const double rout = J->rc + kMargin, rout2 = rout * rout;
+    const double *t = J->t;
     nwf = 0;
     nws = 0;
     for (int k0 = sb; k0 < se; k0 += 64) {
@@ -528,6 +536,13 @@ __device__ __forceinline__ void wave_shell_lists(const JobDev *__restrict__ J,
             const double dx = A.x - wsph.x, dy = A.y - wsph.y, dz = A.z - wsph.z;
             const double d2 = dx * dx + dy * dy + dz * dz;
             cat = d2 < in2 ? (d2 < far2 ? 1 : 2) : 0;
+            if (cat == 2) {
+                const double h = wbox.x * fabs(t[0] * dx + t[3] * dy + t[6] * dz) +
+                                 wbox.y * fabs(t[1] * dx + t[4] * dy + t[7] * dz) +
+                                 wbox.z * fabs(t[2] * dx + t[5] * dy + t[8] * dz);
+                const double g = d2 - h;
+                if (g > 0.0 && g * g > rout2 * d2) cat = 0;
+            }
         }
         const uint64_t bf = __ballot(cat == 1), bs = __ballot(cat == 2);
         const uint16_t off = (uint16_t)((capA - 1 - k) * (int)sizeof(double4));
@@ -558,8 +573,9 @@ __device__ __forceinline__ void wave_shell_lists(const JobDev *__restrict__ J,
 // XL: see pair_r2.
 // ws (MIXED and FP64 grid tiles, wave-uniform): this wave's points lie in the
 // sphere wsph (centre relative to the tile centre, radius w), smaller than
-// the tile's; each segment's shell records are then re-classified against it
-// into the wave's own lists of LDS byte offsets wl (wave_shell_lists).
+// the tile's, and in the parallelepiped wbox around the same centre; each
+// segment's shell records are then re-classified against them into the
+// wave's own lists of LDS byte offsets wl (wave_shell_lists).
 // phase (wave-uniform): 0 every record, per segment far, shell, near; 1 only
 // the near records; 2 only the far and shell records.
 template <int PPL, int MATH, bool XL>
@@ -570,6 +586,7 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                                           const double (&pz)[PPL], double (&acc)[PPL],
                                           double R, bool ws = false,
                                           double4 wsph = make_double4(0.0, 0.0, 0.0, 0.0),
+                                          double4 wbox = make_double4(0.0, 0.0, 0.0, 0.0),
                                           uint16_t *__restrict__ wl = nullptr, int phase = 0) {
     const bool do_fs = phase != 1, do_near = phase != 2;
     const double rc2 = J->rc2, min_r2 = J->min_r2;
@@ -658,7 +675,7 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             if (!do_fs) {
             } else if (ws) {
                 int nwf, nws;
-                wave_shell_lists(J, Ls, capA, capA, sb, se, wsph, wl, nwf, nws);
+                wave_shell_lists(J, Ls, capA, capA, sb, se, wsph, wbox, wl, nwf, nws);
                 const char *__restrict__ Lb = reinterpret_cast<const char *>(L);
                 auto rec = [&](int e) {
                     return *reinterpret_cast<const double4 *>(Lb + wl[e]);
@@ -790,7 +807,7 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             // grids): wave-far ones need no cutoff test, those beyond r_c of
             // every point of the wave contribute exactly 0 and are skipped
             int nwf, nws;
-            wave_shell_lists(J, Ls, capA, wcap, sb, se, wsph, wl, nwf, nws);
+            wave_shell_lists(J, Ls, capA, wcap, sb, se, wsph, wbox, wl, nwf, nws);
             const char *__restrict__ Lb = reinterpret_cast<const char *>(L);
             for (int e = 0; e < nwf; ++e)
                 fp64_pair(*reinterpret_cast<const double4 *>(Lb + wl[e]), no(), no());
@@ -894,7 +911,7 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
     // The sphere around this wave's valid points (quadrant layout): its
     // index box is the tile's i range times the quadrant's j and k ranges.
     bool ws = false, idle = false;
-    double4 wsph = make_double4(0.0, 0.0, 0.0, 0.0);
+    double4 wsph = make_double4(0.0, 0.0, 0.0, 0.0), wbox = wsph;
     if (quad) {
         const int jb = i0[1] + (wave & 1) * h1, kb = i0[2] + (wave >> 1) * h2;
         const int je = jb + h1 - 1 < i1[1] ? jb + h1 - 1 : i1[1];
@@ -905,6 +922,8 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
             const double whi[3] = {hi[0], 1.0 * je / (J->n[1] - 1), 1.0 * ke / (J->n[2] - 1)};
             const Tile w = tile_sphere(J->t, wlo, whi);
             wsph = make_double4(w.cx - tl.cx, w.cy - tl.cy, w.cz - tl.cz, w.R);
+            wbox = make_double4(0.5 * (whi[0] - wlo[0]), 0.5 * (whi[1] - wlo[1]),
+                                0.5 * (whi[2] - wlo[2]), 0.0);
             ws = true;
         }
     }
@@ -916,10 +935,10 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
         if (!idle) {
             if (xl)
                 eval_list<PPL, MATH, true>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R, ws,
-                                           wsph, s_wl);
+                                           wsph, wbox, s_wl);
             else
                 eval_list<PPL, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
-                                            ws, wsph, s_wl);
+                                            ws, wsph, wbox, s_wl);
         }
         __syncthreads();
     } while (cur_t < J->ntypes);
@@ -1159,11 +1178,12 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                 };
                 if (sat && !first && saturated(lb_near + lb_fs + lb_later)) return;
                 eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
-                                          false, make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
+                                          false, make_double4(0.0, 0.0, 0.0, 0.0),
+                                          make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
                 if (!(sat && saturated(lb_fs + lb_later)))
                     eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc,
                                               tl.R, false, make_double4(0.0, 0.0, 0.0, 0.0),
-                                              nullptr, 2);
+                                              make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 2);
 #pragma unroll
                 for (int q = 0; q < P; ++q)
                     if (ok[q]) J->e_out[ins[q]] = first ? acc[q] : prev[q] + acc[q];
@@ -1197,11 +1217,12 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                 };
                 if (sat && !first && saturated(lb_near + lb_fs + lb_later)) return;
                 eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
-                                          false, make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
+                                          false, make_double4(0.0, 0.0, 0.0, 0.0),
+                                          make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
                 if (!(sat && saturated(lb_fs + lb_later)))
                     eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc,
                                               tl.R, false, make_double4(0.0, 0.0, 0.0, 0.0),
-                                              nullptr, 2);
+                                              make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 2);
 #pragma unroll
                 for (int q = 0; q < P; ++q) T[q] = first ? acc[q] : prev[q] + acc[q];
             };
