@@ -298,6 +298,10 @@ enum : int { CAT_NONE = 0, CAT_FAR = 1, CAT_SHELL = 2, CAT_NEAR = 3 };
 // Ints per list segment: type, far begin/end, shell begin/end, near begin/end.
 constexpr int kSegInts = 7;
 
+// Pairs per point summed in FP32 (far and shell pairs of one type) before the
+// sum is scaled and added to the FP64 energy.
+constexpr int kFlushPairs = 16;
+
 struct Tile {
     double cx, cy, cz, R;
 };
@@ -563,7 +567,7 @@ __device__ __forceinline__ void wave_shell_lists(const JobDev *__restrict__ J,
 //   of the tile): r^2 = |a|^2 + |p|^2 - 2 p.a in FP64 (three FMAs and an add;
 //   |a| <= r_c + R keeps the cancellation below 13x), then in FP32
 //   u = 1/r^2, and per point the sums S3 = sum u^3 and S6 = sum u^6 over at
-//   most 8 pairs of one type, flushed into the FP64 energy as
+//   most kFlushPairs pairs of one type, flushed into the FP64 energy as
 //   4 eps (sigma^12 S6 - sigma^6 S3).  sigma and eps stay FP64, so their
 //   rounding adds no error common to every pair of a type.
 //   Shell pairs: the same, behind the strict FP64 test r^2 < r_c^2.
@@ -657,21 +661,33 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                     s6[q] = __builtin_fmaf(w3, w3, s6[q]);
                 }
             };
-            // far records [fb, fe), then shell records [sb, se), 8 pairs per flush
-            for (int r = fb; do_fs && r < fe; r += 8) {
-                float a3[PPL], a6[PPL];
+            // far records [fb, fe), then shell records [sb, se): FP32 sums of
+            // kFlushPairs pairs per point (blocks of 8 unrolled records),
+            // then flushed into the FP64 energy
+            auto blocks = [&](int b, int e, auto &&pair_at) {
+                for (int r = b; r < e; r += kFlushPairs) {
+                    float a3[PPL], a6[PPL];
 #pragma unroll
-                for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
-                if (r + 8 <= fe) {
+                    for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
+                    const int re = r + kFlushPairs < e ? r + kFlushPairs : e;
+#pragma unroll 1
+                    for (int h = r; h < re; h += 8) {
+                        if (h + 8 <= re) {
 #pragma unroll
-                    for (int u = 0; u < 8; ++u) far_pair(L[r + u], a3, a6);
-                } else {
-                    for (int k = r; k < fe; ++k) far_pair(L[k], a3, a6);
+                            for (int u = 0; u < 8; ++u) pair_at(h + u, a3, a6);
+                        } else {
+                            for (int k = h; k < re; ++k) pair_at(k, a3, a6);
+                        }
+                    }
+#pragma unroll
+                    for (int q = 0; q < PPL; ++q)
+                        acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
                 }
-#pragma unroll
-                for (int q = 0; q < PPL; ++q)
-                    acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
-            }
+            };
+            if (do_fs)
+                blocks(fb, fe, [&](int k, float (&s3)[PPL], float (&s6)[PPL]) {
+                    far_pair(L[k], s3, s6);
+                });
             if (!do_fs) {
             } else if (ws) {
                 int nwf, nws;
@@ -680,49 +696,16 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                 auto rec = [&](int e) {
                     return *reinterpret_cast<const double4 *>(Lb + wl[e]);
                 };
-                for (int e0 = 0; e0 < nwf; e0 += 8) {
-                    float a3[PPL], a6[PPL];
-#pragma unroll
-                    for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
-                    if (e0 + 8 <= nwf) {
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) far_pair(rec(e0 + u), a3, a6);
-                    } else {
-                        for (int e = e0; e < nwf; ++e) far_pair(rec(e), a3, a6);
-                    }
-#pragma unroll
-                    for (int q = 0; q < PPL; ++q)
-                        acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
-                }
-                for (int e0 = 0; e0 < nws; e0 += 8) {
-                    float a3[PPL], a6[PPL];
-#pragma unroll
-                    for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
-                    if (e0 + 8 <= nws) {
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) shell_pair(rec(capA - 1 - (e0 + u)), a3, a6);
-                    } else {
-                        for (int e = e0; e < nws; ++e) shell_pair(rec(capA - 1 - e), a3, a6);
-                    }
-#pragma unroll
-                    for (int q = 0; q < PPL; ++q)
-                        acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
-                }
+                blocks(0, nwf, [&](int e, float (&s3)[PPL], float (&s6)[PPL]) {
+                    far_pair(rec(e), s3, s6);
+                });
+                blocks(0, nws, [&](int e, float (&s3)[PPL], float (&s6)[PPL]) {
+                    shell_pair(rec(capA - 1 - e), s3, s6);
+                });
             } else {
-                for (int k0 = sb; k0 < se; k0 += 8) {
-                    float a3[PPL], a6[PPL];
-#pragma unroll
-                    for (int q = 0; q < PPL; ++q) a3[q] = a6[q] = 0.0f;
-                    if (k0 + 8 <= se) {
-#pragma unroll
-                        for (int u = 0; u < 8; ++u) shell_pair(Ls[-(k0 + u)], a3, a6);
-                    } else {
-                        for (int k = k0; k < se; ++k) shell_pair(Ls[-k], a3, a6);
-                    }
-#pragma unroll
-                    for (int q = 0; q < PPL; ++q)
-                        acc[q] = fma(c6, (double)a6[q], fma(c3, (double)a3[q], acc[q]));
-                }
+                blocks(sb, se, [&](int k, float (&s3)[PPL], float (&s6)[PPL]) {
+                    shell_pair(Ls[-k], s3, s6);
+                });
             }
             if (!do_near) continue;
             // Near pairs of an FP32 grid or Widom run: the expanded FP64 r^2 of
