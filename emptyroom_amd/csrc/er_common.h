@@ -101,6 +101,11 @@ struct JobDev {
     double *e_out;      // FP64 energy per insertion index
     double sat_e;       // > 0: stop at a partial energy this high (800 T; see k_tiles)
     double eps_max;     // largest eps_ij of the framework's pair types
+    // point boxes (fractional half-extents per axis): a full grid tile's or a
+    // bin's, and a full wave quadrant's of a grid tile; a partial one lies in
+    // the full one centred on it
+    double cell_box[3];
+    double wave_box[3];
     // scheduling
     long long item_begin;  // first global work item of this job
     long long n_items;     // grid tiles or Widom bins of this job in this launch

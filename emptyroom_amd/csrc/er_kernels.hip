@@ -366,6 +366,18 @@ __device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict
     // FP64 grids: near (clamp possible) only if some point may lie within
     // kMargin of the atom, d <= R + kMargin
     const double clamp2 = (tl.R + kMargin) * (tl.R + kMargin);
+    // The cell's points lie in a parallelepiped around the centre:
+    // J->cell_box[d] times cell vector t_d on either side (a bin's box; a
+    // full tile's, which holds a partial edge tile's around that tile's own
+    // centre).  A shell record at v from the centre with |v|^2 - h(v) >
+    // (r_c + m)|v|, h(v) = sum_d cell_box_d |t_d . v|, is beyond r_c of every
+    // point (|a - x| >= |v| - h(v)/|v|) and adds exactly 0: it is not kept.
+    // (FP32 and FP64 grid tiles re-test their shell records per wave, against
+    // the wave's smaller box, which makes this test redundant there)
+    const bool wave_lists = J->kind == JOB_GRID && (MATH == MATH_MIXED || MATH == MATH_FP64);
+    const double *hb = J->cell_box;
+    const double rout = rc + kMargin, rout2 = rout * rout;
+    const double *tm = J->t;
     double near2 = -1.0;
     int near_t = -1;
     int nf = 0, ns = 0, nn = 0, nseg = 0, f0 = 0, s0 = 0, n0 = 0;
@@ -404,6 +416,13 @@ __device__ int fill_list(const JobDev *__restrict__ J, const double4 *__restrict
                 if (MATH == MATH_FP64 && d2 <= clamp2) cat = CAT_NEAR;
                 else if (MATH == MATH_MIXED && d2 <= near2) cat = CAT_NEAR;
                 else cat = (MATH != MATH_OVERLAP && d2 < far2) ? CAT_FAR : CAT_SHELL;
+                if (cat == CAT_SHELL && !wave_lists) {
+                    const double h = hb[0] * fabs(tm[0] * dx + tm[3] * dy + tm[6] * dz) +
+                                     hb[1] * fabs(tm[1] * dx + tm[4] * dy + tm[7] * dz) +
+                                     hb[2] * fabs(tm[2] * dx + tm[5] * dy + tm[8] * dz);
+                    const double g = d2 - h;
+                    if (g > 0.0 && g * g > rout2 * d2) cat = CAT_NONE;
+                }
             }
         }
         const uint64_t bf = __ballot(cat == CAT_FAR);
@@ -511,25 +530,26 @@ __device__ __forceinline__ void pair_r2(const double4 &A, const double (&mx)[PPL
 // wave's sphere wsph (centre relative to the tile centre, radius wsph.w):
 // records beyond r_c + w of it are out of the cutoff for all of the wave's
 // points and dropped, those within r_c - w are inside it for all of them (no
-// cutoff test).  The wave's points fill a parallelepiped around that centre,
-// wbox.{x,y,z} times the cell vectors on either side; for a record at v from
-// the centre every point x has |a - x| >= (a - x).v/|v| >= |v| - h(v)/|v|,
-// h(v) = sum_d wbox_d |t_d . v| (t_d the cell vectors), so a record with
+// cutoff test).  The wave's points lie in a parallelepiped around that
+// centre, J->wave_box[d] times cell vector t_d on either side; for a record
+// at v from the centre every point x has |a - x| >= (a - x).v/|v| >=
+// |v| - h(v)/|v|, h(v) = sum_d wave_box_d |t_d . v|, so a record with
 // |v|^2 - h(v) > (r_c + m)|v| is out of the cutoff for every point as well
-// and is dropped: it added exactly 0 to every sum.  The wave writes its own
+// and is dropped: it added exactly 0 to every sum.  (The matching bound
+// |a - x|^2 <= |v|^2 + 2 h(v) + w^2 would move a few records from the
+// shell to the far list; it measured no gain.)  The wave writes its own
 // list of LDS byte offsets (from the start of the list, capA - 1 - k
 // records), order kept: the nwf wave-far ones from the front of wl, the nws
 // still-shell ones from wl[wcap - 1] down (se - sb <= wcap entries).
 __device__ __forceinline__ void wave_shell_lists(const JobDev *__restrict__ J,
                                                  const double4 *__restrict__ Ls, int capA,
                                                  int wcap, int sb, int se, const double4 &wsph,
-                                                 const double4 &wbox, uint16_t *__restrict__ wl,
-                                                 int &nwf, int &nws) {
+                                                 uint16_t *__restrict__ wl, int &nwf, int &nws) {
     const int lane = threadIdx.x & 63;
     const double rin = J->rc + wsph.w + kMargin, rfar = J->rc - wsph.w - kMargin;
     const double in2 = rin * rin, far2 = rfar > 0.0 ? rfar * rfar : -1.0;
     const double rout = J->rc + kMargin, rout2 = rout * rout;
-    const double *t = J->t;
+    const double *t = J->t, *wb = J->wave_box;
     nwf = 0;
     nws = 0;
     for (int k0 = sb; k0 < se; k0 += 64) {
@@ -541,9 +561,9 @@ __device__ __forceinline__ void wave_shell_lists(const JobDev *__restrict__ J,
             const double d2 = dx * dx + dy * dy + dz * dz;
             cat = d2 < in2 ? (d2 < far2 ? 1 : 2) : 0;
             if (cat == 2) {
-                const double h = wbox.x * fabs(t[0] * dx + t[3] * dy + t[6] * dz) +
-                                 wbox.y * fabs(t[1] * dx + t[4] * dy + t[7] * dz) +
-                                 wbox.z * fabs(t[2] * dx + t[5] * dy + t[8] * dz);
+                const double h = wb[0] * fabs(t[0] * dx + t[3] * dy + t[6] * dz) +
+                                 wb[1] * fabs(t[1] * dx + t[4] * dy + t[7] * dz) +
+                                 wb[2] * fabs(t[2] * dx + t[5] * dy + t[8] * dz);
                 const double g = d2 - h;
                 if (g > 0.0 && g * g > rout2 * d2) cat = 0;
             }
@@ -577,9 +597,9 @@ __device__ __forceinline__ void wave_shell_lists(const JobDev *__restrict__ J,
 // XL: see pair_r2.
 // ws (MIXED and FP64 grid tiles, wave-uniform): this wave's points lie in the
 // sphere wsph (centre relative to the tile centre, radius w), smaller than
-// the tile's, and in the parallelepiped wbox around the same centre; each
-// segment's shell records are then re-classified against them into the
-// wave's own lists of LDS byte offsets wl (wave_shell_lists).
+// the tile's, and in the parallelepiped of J->wave_box around the same
+// centre; each segment's shell records are then re-classified against them
+// into the wave's own lists of LDS byte offsets wl (wave_shell_lists).
 // phase (wave-uniform): 0 every record, per segment far, shell, near; 1 only
 // the near records; 2 only the far and shell records.
 template <int PPL, int MATH, bool XL>
@@ -590,7 +610,6 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
                                           const double (&pz)[PPL], double (&acc)[PPL],
                                           double R, bool ws = false,
                                           double4 wsph = make_double4(0.0, 0.0, 0.0, 0.0),
-                                          double4 wbox = make_double4(0.0, 0.0, 0.0, 0.0),
                                           uint16_t *__restrict__ wl = nullptr, int phase = 0) {
     const bool do_fs = phase != 1, do_near = phase != 2;
     const double rc2 = J->rc2, min_r2 = J->min_r2;
@@ -691,7 +710,7 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             if (!do_fs) {
             } else if (ws) {
                 int nwf, nws;
-                wave_shell_lists(J, Ls, capA, capA, sb, se, wsph, wbox, wl, nwf, nws);
+                wave_shell_lists(J, Ls, capA, capA, sb, se, wsph, wl, nwf, nws);
                 const char *__restrict__ Lb = reinterpret_cast<const char *>(L);
                 auto rec = [&](int e) {
                     return *reinterpret_cast<const double4 *>(Lb + wl[e]);
@@ -790,7 +809,7 @@ __device__ __forceinline__ void eval_list(const JobDev *__restrict__ J,
             // grids): wave-far ones need no cutoff test, those beyond r_c of
             // every point of the wave contribute exactly 0 and are skipped
             int nwf, nws;
-            wave_shell_lists(J, Ls, capA, wcap, sb, se, wsph, wbox, wl, nwf, nws);
+            wave_shell_lists(J, Ls, capA, wcap, sb, se, wsph, wl, nwf, nws);
             const char *__restrict__ Lb = reinterpret_cast<const char *>(L);
             for (int e = 0; e < nwf; ++e)
                 fp64_pair(*reinterpret_cast<const double4 *>(Lb + wl[e]), no(), no());
@@ -894,7 +913,7 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
     // The sphere around this wave's valid points (quadrant layout): its
     // index box is the tile's i range times the quadrant's j and k ranges.
     bool ws = false, idle = false;
-    double4 wsph = make_double4(0.0, 0.0, 0.0, 0.0), wbox = wsph;
+    double4 wsph = make_double4(0.0, 0.0, 0.0, 0.0);
     if (quad) {
         const int jb = i0[1] + (wave & 1) * h1, kb = i0[2] + (wave >> 1) * h2;
         const int je = jb + h1 - 1 < i1[1] ? jb + h1 - 1 : i1[1];
@@ -905,8 +924,6 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
             const double whi[3] = {hi[0], 1.0 * je / (J->n[1] - 1), 1.0 * ke / (J->n[2] - 1)};
             const Tile w = tile_sphere(J->t, wlo, whi);
             wsph = make_double4(w.cx - tl.cx, w.cy - tl.cy, w.cz - tl.cz, w.R);
-            wbox = make_double4(0.5 * (whi[0] - wlo[0]), 0.5 * (whi[1] - wlo[1]),
-                                0.5 * (whi[2] - wlo[2]), 0.0);
             ws = true;
         }
     }
@@ -918,10 +935,10 @@ __device__ void grid_tile(const JobDev *__restrict__ J, const int tc[3],
         if (!idle) {
             if (xl)
                 eval_list<PPL, MATH, true>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R, ws,
-                                           wsph, wbox, s_wl);
+                                           wsph, s_wl);
             else
                 eval_list<PPL, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
-                                            ws, wsph, wbox, s_wl);
+                                            ws, wsph, s_wl);
         }
         __syncthreads();
     } while (cur_t < J->ntypes);
@@ -1161,12 +1178,11 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                 };
                 if (sat && !first && saturated(lb_near + lb_fs + lb_later)) return;
                 eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
-                                          false, make_double4(0.0, 0.0, 0.0, 0.0),
-                                          make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
+                                          false, make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
                 if (!(sat && saturated(lb_fs + lb_later)))
                     eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc,
                                               tl.R, false, make_double4(0.0, 0.0, 0.0, 0.0),
-                                              make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 2);
+                                              nullptr, 2);
 #pragma unroll
                 for (int q = 0; q < P; ++q)
                     if (ok[q]) J->e_out[ins[q]] = first ? acc[q] : prev[q] + acc[q];
@@ -1200,12 +1216,11 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                 };
                 if (sat && !first && saturated(lb_near + lb_fs + lb_later)) return;
                 eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
-                                          false, make_double4(0.0, 0.0, 0.0, 0.0),
-                                          make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
+                                          false, make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
                 if (!(sat && saturated(lb_fs + lb_later)))
                     eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc,
                                               tl.R, false, make_double4(0.0, 0.0, 0.0, 0.0),
-                                              make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 2);
+                                              nullptr, 2);
 #pragma unroll
                 for (int q = 0; q < P; ++q) T[q] = first ? acc[q] : prev[q] + acc[q];
             };

@@ -546,6 +546,11 @@ JobDev make_grid_job(const FrameworkDev *fw, const int n[3], int precision, void
         j.region[d] = std::min(region[d], j.ntiles[d]);
         j.nregions[d] = (j.ntiles[d] + j.region[d] - 1) / j.region[d];
         items *= j.nregions[d];
+        // point boxes; the quadrant of a tile holds its i range and half its
+        // j and k ranges (used only where the kernel takes that layout)
+        const int wave_pts = d == 0 ? tile[d] : std::max(tile[d] / 2, 1);
+        j.cell_box[d] = 0.5 * (double)(tile[d] - 1) / (double)(n[d] - 1);
+        j.wave_box[d] = 0.5 * (double)(wave_pts - 1) / (double)(n[d] - 1);
     }
     j.out = d_out;
     j.n_items = items;
@@ -567,7 +572,10 @@ JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
     j.ntypes = fw->ntypes;
     j.kind = JOB_WIDOM;
     j.mode = OUT_F64;
-    for (int d = 0; d < 3; d++) j.nbins[d] = wb.nbins[d];
+    for (int d = 0; d < 3; d++) {
+        j.nbins[d] = wb.nbins[d];
+        j.cell_box[d] = 0.5 / (double)wb.nbins[d];
+    }
     j.sorted = wb.sorted;
     j.bin_start = wb.start;
     j.seed = seed;
