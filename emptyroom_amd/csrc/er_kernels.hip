@@ -10,15 +10,17 @@
 //   types   : TypeParams[ntypes]   mixed sigma^2, 4 eps, FP64 switch radius
 // The tiled kernel (k_tiles) is persistent: each workgroup takes work items
 // (a region of grid tiles or of Widom bins) from a device counter, gathers the
-// region's candidate sub-list into its scratch slice, then per tile (a box of
-// grid points, or one bin's insertions) keeps the image atoms within
-// r_c + R_tile in LDS, split into far records (inside the cutoff and beyond
-// the FP64 switch radius for every point: no test, FP32 reciprocal powers),
-// shell records (cutoff test) and near records (FP64 pair arithmetic).
-// Every lane evaluates its points against the LDS list with the record
-// broadcast from LDS and the pair parameters in registers; in FP32 grid
-// tiles each wave first re-sorts the shell records against its own, smaller
-// sphere.
+// near-list indices of the region's candidates into its scratch slice, then
+// per tile (a box of grid points, or one bin's insertions) keeps the image
+// atoms within r_c + R_tile in LDS, split into far records (inside the cutoff
+// and beyond the FP64 switch radius for every point: no test, FP32
+// reciprocal powers), shell records (cutoff test) and near records (FP64
+// pair arithmetic).  Every lane evaluates its points against the LDS list
+// with the record broadcast from LDS and the pair parameters in registers;
+// in FP32 and FP64 grid tiles each wave first re-sorts the shell records
+// against its own, smaller point box.  A Widom bin of at most one chunk of
+// insertions keeps their points and partial energies in registers across
+// its LDS passes.
 #include "er_kernels.h"
 
 #include <float.h>
