@@ -1150,10 +1150,46 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             // saturated stops evaluating them; the stored partial energy
             // (>= 800 T) then stands in for the exact one.
             constexpr int PW = PPL > 2 ? 2 : PPL;
+            // One pass of this wave's P points (px.., valid where ok) over the
+            // LDS list: the near records, then, unless every valid point is
+            // saturated by then, the far and shell records.  out = prev + the
+            // pass's terms (the terms alone in the first pass).  Returns false,
+            // out untouched, if the points were saturated before the pass:
+            // their stored energies stand.
+            auto one_pass = [&](auto pw, const auto &px, const auto &py, const auto &pz,
+                                const auto &ok, const auto &prev, auto &out, const Tile &tl,
+                                int nseg, bool first, double lb_near, double lb_fs,
+                                double lb_later) {
+                constexpr int P = decltype(pw)::value;
+                double acc[P];
+#pragma unroll
+                for (int q = 0; q < P; ++q) acc[q] = 0.0;
+                const bool sat = J->sat_e > 0.0;
+                auto saturated = [&](double lb) {
+                    bool all = true;
+#pragma unroll
+                    for (int q = 0; q < P; ++q)
+                        if (ok[q]) all = all && prev[q] + acc[q] + lb >= J->sat_e;
+                    return __all(all) != 0;
+                };
+                if (sat && !first && saturated(lb_near + lb_fs + lb_later)) return false;
+                eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
+                                          false, make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
+                if (!(sat && saturated(lb_fs + lb_later)))
+                    eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc,
+                                              tl.R, false, make_double4(0.0, 0.0, 0.0, 0.0),
+                                              nullptr, 2);
+#pragma unroll
+                for (int q = 0; q < P; ++q) out[q] = first ? acc[q] : prev[q] + acc[q];
+                return true;
+            };
+            // A chunk of up to P * 256 of the bin's insertions, slots
+            // c + q * 256 + tid: points generated and energies loaded and
+            // stored for this pass.
             auto chunk = [&](auto pw, uint64_t c, uint64_t end, const Tile &tl, int nseg,
                              bool first, double lb_near, double lb_fs, double lb_later) {
                 constexpr int P = decltype(pw)::value;
-                double px[P], py[P], pz[P], acc[P], prev[P];
+                double px[P], py[P], pz[P], prev[P], out[P];
                 uint32_t ins[P];
                 bool ok[P];
 #pragma unroll
@@ -1167,64 +1203,40 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                     px[q] -= tl.cx;
                     py[q] -= tl.cy;
                     pz[q] -= tl.cz;
-                    acc[q] = 0.0;
                     prev[q] = first ? 0.0 : J->e_out[ins[q]];
                 }
-                const bool sat = J->sat_e > 0.0;
-                auto saturated = [&](double lb) {
-                    bool all = true;
-#pragma unroll
-                    for (int q = 0; q < P; ++q)
-                        if (ok[q]) all = all && prev[q] + acc[q] + lb >= J->sat_e;
-                    return __all(all) != 0;
-                };
-                if (sat && !first && saturated(lb_near + lb_fs + lb_later)) return;
-                eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
-                                          false, make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
-                if (!(sat && saturated(lb_fs + lb_later)))
-                    eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc,
-                                              tl.R, false, make_double4(0.0, 0.0, 0.0, 0.0),
-                                              nullptr, 2);
+                if (!one_pass(pw, px, py, pz, ok, prev, out, tl, nseg, first, lb_near, lb_fs,
+                              lb_later))
+                    return;
 #pragma unroll
                 for (int q = 0; q < P; ++q)
-                    if (ok[q]) J->e_out[ins[q]] = first ? acc[q] : prev[q] + acc[q];
+                    if (ok[q]) J->e_out[ins[q]] = out[q];
             };
             // A bin whose insertions fit one chunk (at most PW * 256, most
             // bins) keeps each lane's points and partial energies in
             // registers across the passes and writes the energies once; the
             // passes add the same terms in the same order as the chunked
-            // form below, which stores and reloads them after every pass.
+            // form, which stores and reloads them after every pass.
             auto resident = [&](auto pw, const double (&X)[PW], const double (&Y)[PW],
                                 const double (&Z)[PW], const bool (&OK)[PW], double (&T)[PW],
                                 const Tile &tl, int nseg, bool first, double lb_near,
                                 double lb_fs, double lb_later) {
                 constexpr int P = decltype(pw)::value;
-                double px[P], py[P], pz[P], acc[P], prev[P];
+                double px[P], py[P], pz[P], prev[P], out[P];
+                bool ok[P];
 #pragma unroll
                 for (int q = 0; q < P; ++q) {
                     px[q] = X[q];
                     py[q] = Y[q];
                     pz[q] = Z[q];
-                    acc[q] = 0.0;
+                    ok[q] = OK[q];
                     prev[q] = first ? 0.0 : T[q];
                 }
-                const bool sat = J->sat_e > 0.0;
-                auto saturated = [&](double lb) {
-                    bool all = true;
+                if (!one_pass(pw, px, py, pz, ok, prev, out, tl, nseg, first, lb_near, lb_fs,
+                              lb_later))
+                    return;
 #pragma unroll
-                    for (int q = 0; q < P; ++q)
-                        if (OK[q]) all = all && prev[q] + acc[q] + lb >= J->sat_e;
-                    return __all(all) != 0;
-                };
-                if (sat && !first && saturated(lb_near + lb_fs + lb_later)) return;
-                eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc, tl.R,
-                                          false, make_double4(0.0, 0.0, 0.0, 0.0), nullptr, 1);
-                if (!(sat && saturated(lb_fs + lb_later)))
-                    eval_list<P, MATH, false>(J, s_list, cap, s_seg, nseg, px, py, pz, acc,
-                                              tl.R, false, make_double4(0.0, 0.0, 0.0, 0.0),
-                                              nullptr, 2);
-#pragma unroll
-                for (int q = 0; q < P; ++q) T[q] = first ? acc[q] : prev[q] + acc[q];
+                for (int q = 0; q < P; ++q) T[q] = out[q];
             };
             const int b = (tc[0] * J->nbins[1] + tc[1]) * J->nbins[2] + tc[2];
             const uint32_t s0 = J->bin_start[b], s1 = J->bin_start[b + 1];
