@@ -1142,11 +1142,13 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             // its far and shell records, to the insertion's energy.
             //
             // Saturation (J->sat_e > 0, jobs whose energies only feed the
-            // Boltzmann sums): no pair energy is below -eps_ij, so once an
-            // insertion's partial energy plus -eps_ij for every record not
-            // yet evaluated reaches sat_e = 800 T, its exact energy is at
-            // least 800 T as well, exp(-E/T) is exactly 0 in FP64 for both,
-            // and neither sum changes.  A wave whose valid insertions are all
+            // Boltzmann sums): no pair energy is below -eps_ij, and none of a
+            // far or shell record (beyond r_FP64 of every point) below
+            // -fs_floor eps_ij, so once an insertion's partial energy plus
+            // those floors for every record not yet evaluated (-eps_ij for
+            // records not yet classified) reaches sat_e = 800 T, its exact
+            // energy is at least 800 T as well, exp(-E/T) is exactly 0 in
+            // FP64 for both, and neither sum changes.  A wave whose valid insertions are all
             // saturated stops evaluating them; the stored partial energy
             // (>= 800 T) then stands in for the exact one.
             constexpr int PW = PPL > 2 ? 2 : PPL;
@@ -1287,7 +1289,8 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                     for (int sg = 0; sg < nseg; ++sg) {
                         const int *g = s_seg + kSegInts * sg;
                         const double eps = 0.25 * J->types[uniform(g[0])].eps4;
-                        lb_fs -= eps * (double)(uniform(g[2]) - uniform(g[1]) + uniform(g[4]) -
+                        lb_fs -= eps * J->fs_floor *
+                                 (double)(uniform(g[2]) - uniform(g[1]) + uniform(g[4]) -
                                                 uniform(g[3]));
                         lb_near -= eps * (double)(uniform(g[6]) - uniform(g[5]));
                     }

@@ -301,6 +301,7 @@ int framework_upload(Context *ctx, const Particle *atoms, int n, const double t[
     fw->rc2 = rc2;
     fw->rc = std::sqrt(rc2);
     fw->volume = std::fabs(det);
+    fw->s2_switch = ctx->tuning.s2_switch;
     std::memcpy(fw->rep, rep, sizeof(fw->rep));
     fw->n_images = (long long)(2 * rep[0] + 1) * (2 * rep[1] + 1) * (2 * rep[2] + 1);
 
@@ -590,6 +591,15 @@ JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
     }
     j.sat_e = (temperature > 0.0 && std::isfinite(temperature) && eps_ok) ? 800.0 * temperature
                                                                          : 0.0;
+    // A far or shell pair lies beyond r_FP64 = sigma / sqrt(s2) of its point,
+    // u = (sigma / r)^2 <= s2: 4 eps (u^6 - u^3) decreases in u up to
+    // u = 2^(-1/3) (the LJ minimum, -eps), so it is at least 4 eps (s2^6 - s2^3)
+    // below that and -eps beyond; 0.01 eps of margin covers the FP32 pair math.
+    const double s2 = fw->s2_switch;
+    const double fmin = (s2 > 0.0 && s2 < 0.7937005259840998)
+                            ? 4.0 * (s2 * s2 * s2 * s2 * s2 * s2 - s2 * s2 * s2)
+                            : -1.0;
+    j.fs_floor = std::min(1.0, -fmin + 0.01);
     // regions of bins share one candidate sub-list, like regions of grid tiles
     double frac[3];
     for (int d = 0; d < 3; d++) frac[d] = 1.0 / wb.nbins[d];

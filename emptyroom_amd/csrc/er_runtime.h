@@ -119,6 +119,7 @@ struct FrameworkDev {
     Context *ctx = nullptr;
     int n_atoms = 0, ntypes = 0;
     double t[9], inv_t[9], rc2 = 0, rc = 0, volume = 0;
+    double s2_switch = 0;  // (sigma / r_FP64)^2 the types' switch radii were built with
     int rep[3] = {0, 0, 0};
     long long n_images = 0;
     size_t near_capacity = 0;
