@@ -100,7 +100,6 @@ struct JobDev {
     uint32_t pad2;
     double *e_out;      // FP64 energy per insertion index
     double sat_e;       // > 0: stop at a partial energy this high (800 T; see k_tiles)
-    double eps_max;     // largest eps_ij of the framework's pair types
     double fs_floor;    // far and shell pairs (beyond r_FP64 of every point) add at least
                         // -fs_floor * eps_ij each (<= 1)
     // point boxes (fractional half-extents per axis): a full grid tile's or a

@@ -1145,8 +1145,8 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
             // Boltzmann sums): no pair energy is below -eps_ij, and none of a
             // far or shell record (beyond r_FP64 of every point) below
             // -fs_floor eps_ij, so once an insertion's partial energy plus
-            // those floors for every record not yet evaluated (-eps_ij for
-            // records not yet classified) reaches sat_e = 800 T, its exact
+            // those floors for every record not yet evaluated (-eps_ij of its
+            // type for records not yet classified) reaches sat_e = 800 T, its exact
             // energy is at least 800 T as well, exp(-E/T) is exactly 0 in
             // FP64 for both, and neither sum changes.  A wave whose valid insertions are all
             // saturated stops evaluating them; the stored partial energy
@@ -1294,8 +1294,15 @@ __global__ __launch_bounds__(kWG) void k_tiles(const JobDev *__restrict__ jobs, 
                                                 uniform(g[3]));
                         lb_near -= eps * (double)(uniform(g[6]) - uniform(g[5]));
                     }
-                    if (cur_t < J->ntypes)
-                        lb_later = -J->eps_max * (double)(uniform(tbeg[J->ntypes]) - cur_pos);
+                    // source records after this pass, each at -eps of its type:
+                    // the rest of the current type, then every later type
+                    if (cur_t < J->ntypes) {
+                        lb_later = -0.25 * J->types[cur_t].eps4 *
+                                   (double)(uniform(tbeg[cur_t + 1]) - cur_pos);
+                        for (int u = cur_t + 1; u < J->ntypes; ++u)
+                            lb_later -= 0.25 * J->types[u].eps4 *
+                                        (double)(uniform(tbeg[u + 1]) - uniform(tbeg[u]));
+                    }
                 }
                 if (mode == PW)
                     resident(std::integral_constant<int, PW>(), rx, ry, rz, rok, rt, tl, nseg,

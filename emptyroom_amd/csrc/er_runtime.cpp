@@ -585,10 +585,7 @@ JobDev make_widom_job(const FrameworkDev *fw, uint64_t n_ins, uint64_t seed,
     // saturation bound: every pair energy is >= -eps_ij when no eps_ij is
     // negative (exp(-800) is exactly 0 in FP64)
     bool eps_ok = true;
-    for (const TypeParams &tp : fw->types) {
-        eps_ok = eps_ok && tp.eps4 >= 0.0;
-        j.eps_max = std::max(j.eps_max, 0.25 * tp.eps4);
-    }
+    for (const TypeParams &tp : fw->types) eps_ok = eps_ok && tp.eps4 >= 0.0;
     j.sat_e = (temperature > 0.0 && std::isfinite(temperature) && eps_ok) ? 800.0 * temperature
                                                                          : 0.0;
     // A far or shell pair lies beyond r_FP64 = sigma / sqrt(s2) of its point,
